@@ -1,0 +1,181 @@
+#!/usr/bin/env python3
+"""Headline benchmark: Gf8D (0x11D) RS(10+4) encode on one MI355X.
+
+One step = one `dot_product_matrix_with` over k = 10 source shards of
+L = 107,374,144 bytes (about 1 GiB of source in total) into m = 4 parity rows,
+Cauchy coefficients 1/(x_i + y_j) with x_i = i, y_j = 4 + j, seeded uniform
+source bytes, everything resident in HBM before the timed region. The working
+set (1.5 GB) is far larger than the 256 MB Infinity Cache, so the timing is an
+HBM measurement.
+
+Prints one JSON line: throughput over algorithmic bytes (k + m) * L per step,
+the roofline of the encode kernel (HIP events around the timed launches on
+their stream, cross-checked by the rocprofv3 summary under profiles/), and
+the timed CPU baseline (a C port of the reference's dispatched x86 path,
+oracle/cpu_port.c) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+K, M = 10, 4
+L_HEADLINE = ((1 << 30) // K) & ~63  # 107,374,144 B: 1 GiB of source rounded to 64 B
+HBM_PEAK_GBS = 8000.0
+METRIC = "RS(10+4) GF(2^8) encode GB/s & %HBM peak; batched-serving p50/p99 latency"
+
+
+def cauchy_plan_values(k: int, m: int) -> list[int]:
+    """Plan (k, m): coefficient (t, j) = 1 / (j ^ (m + t)) in GF(2^8)/0x11D."""
+    from fgf_amd import Gf8D
+    return [Gf8D.inv(j ^ (m + t)) for t in range(k) for j in range(m)]
+
+
+def cpu_baseline(seconds_budget: float = 12.0) -> dict:
+    """Time the C port of the reference's x86 Gf8D matrix path on host cores.
+
+    Sample: RS(10+4) with 16 MiB shards (the headline shape at 1/6.4 of the
+    row length), timed like benches/kernels.rs:41-64 (warm-up, then the
+    median of repeated runs), first on 1 core (the reference is
+    single-threaded per call), then on all cores.
+    """
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+
+    import oracle_lib as ol
+    from fgf_amd import Gf8D  # noqa: F401
+
+    L = 16 << 20
+    rng = np.random.default_rng(1)
+    srcs = [rng.integers(0, 256, L, dtype=np.uint8) for _ in range(K)]
+    coeffs = np.array(cauchy_plan_values(K, M), dtype=np.uint8)
+    rows = np.zeros(M * L, dtype=np.uint8)
+    backend = 0 if ol.has_gfni() else 1
+    name = "v3_gfni_crypto" if backend == 0 else "v3"
+    results = {}
+    ncores = len(os.sched_getaffinity(0))
+    for threads in (1, ncores):
+        assert ol.cpu_port_dot_product_matrix_8d(backend, rows, L, M, coeffs, srcs, threads) == 0
+        samples = []
+        t_end = time.perf_counter() + seconds_budget / 2
+        while len(samples) < 64 and (time.perf_counter() < t_end or len(samples) < 3):
+            t0 = time.perf_counter()
+            ol.cpu_port_dot_product_matrix_8d(backend, rows, L, M, coeffs, srcs, threads)
+            samples.append(time.perf_counter() - t0)
+        med = float(np.median(samples))
+        results[threads] = ((K + M) * L / med / 1e9, K * L / med / 2**30)
+    # bit-exact check of the timed output against the scalar oracle on a slice
+    want = np.zeros(M * 4096, dtype=np.uint8)
+    ol.mul_add_matrix("gf8d", want, 4096, M, [int(c) for c in coeffs], [s[:4096] for s in srcs], True)
+    got = rows.reshape(M, L)[:, :4096].reshape(-1)
+    assert np.array_equal(got, want), "CPU baseline output differs from the oracle"
+    return {
+        "value": round(results[1][0], 2),
+        "unit": "GB/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"RS(10+4) Gf8D encode, 10 x 16 MiB shards, median of repeated runs; "
+                  f"C port of the reference's {name} path (oracle/cpu_port.c)",
+        "all_cores": {"cores": ncores, "value": round(results[ncores][0], 2), "unit": "GB/s"},
+        "source_GiBps_1core": round(results[1][1], 2),
+        "backend": name,
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--row-len", type=int, default=L_HEADLINE)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              f"(the sharding design is documented, multi-GPU runs are not part of it)",
+              file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from fgf_amd import Gf8D, ops
+
+    dev = torch.device("cuda:0")
+    L = args.row_len
+    print(f"[bench] RS({K}+{M}) Gf8D encode, L={L} B per shard", file=sys.stderr, flush=True)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(20240601)
+    src = torch.randint(0, 256, (K, L), dtype=torch.uint8, device=dev, generator=gen)
+    rows = torch.empty((M, L), dtype=torch.uint8, device=dev)
+    srcs = [src[t] for t in range(K)]
+    plan = ops.Plan.matrix(Gf8D, K, M, cauchy_plan_values(K, M))
+    stream = torch.cuda.current_stream(dev)
+
+    for _ in range(args.warmup):
+        ops.dot_product_matrix_with(Gf8D, rows.view(-1), L, M, plan, srcs)
+    torch.cuda.synchronize()
+
+    # Correctness spot check of this exact launch against the oracle.
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    probe = 1 << 16
+    want = np.zeros(M * probe, dtype=np.uint8)
+    ol.mul_add_matrix("gf8d", want, probe, M, cauchy_plan_values(K, M),
+                      [src[t, -probe:].cpu().numpy() for t in range(K)], True)
+    assert np.array_equal(rows[:, -probe:].cpu().numpy().reshape(-1), want), "encode mismatch"
+
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        starts[i].record(stream)
+        ops.dot_product_matrix_with(Gf8D, rows.view(-1), L, M, plan, srcs)
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    avg_ms = sum(kern_ms) / len(kern_ms)
+    bytes_per_step = (K + M) * L
+    ms_per_step = wall * 1e3 / args.steps
+    value = bytes_per_step / (ms_per_step * 1e-3) / 1e9
+    achieved = bytes_per_step / (avg_ms * 1e-3) / 1e9
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic: seeded uniform random source bytes, Cauchy coefficients",
+        "config": {"workload": "Gf8D RS(10+4) systematic encode via dot_product_matrix_with",
+                   "k": K, "m": M, "shard_bytes": L, "source_bytes": K * L,
+                   "bytes_per_step": bytes_per_step, "parallelism": "single GPU"},
+        "source_GiBps": round(K * L / (ms_per_step * 1e-3) / 2**30, 2),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel_avg_ms": round(avg_ms, 5),
+                     "kernel": "k_matrix<B=1,M=4,D=8,overwrite>"},
+    }
+    if not args.no_cpu:
+        print("[bench] CPU baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,846 @@
+// C ABI over the HIP kernels: the reference's `ops` façade (src/ops.rs),
+// with its geometry checks, short-circuits and panic messages reproduced
+// exactly. A misuse returns FGF_EINVAL before anything is enqueued, with the
+// reference's message in fgf_last_error().
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/fgf_amd.h"
+#include "field.hpp"
+#include "kernels.hpp"
+
+struct fgf_plan {
+    int field = 0;
+    int nb = 1;
+    size_t rows = 0, cols = 0;
+    std::vector<uint64_t> values;
+    uint32_t *d_tab = nullptr;  // entry i at i * nb * nb * TAB_DW dwords
+    int device = 0;
+};
+
+namespace {
+
+using namespace fgf;
+
+thread_local std::string g_err;
+
+int fail(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+int fail(const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return FGF_EINVAL;
+}
+
+int device_fail(const char *what, hipError_t e) {
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return FGF_EDEVICE;
+}
+
+size_t entry_dw(int nb) { return (size_t)nb * nb * TAB_DW; }
+
+bool field_ok(int f) { return valid_field(f); }
+
+uint64_t width_mask(int f) {
+    const int nb = field_bytes(f);
+    return nb == 8 ? ~0ull : ((1ull << (8 * nb)) - 1);
+}
+
+uint64_t read_elem(const uint8_t *p, int nb) {
+    uint64_t v = 0;
+    for (int i = 0; i < nb; ++i) v |= (uint64_t)p[i] << (8 * i);
+    return v;
+}
+
+// ops.rs:300-307
+bool check_width(int f, const char *name, size_t len, int *rc) {
+    if (len % (size_t)field_bytes(f) != 0) {
+        *rc = fail("%s: buffer of %zu bytes is not a whole number of %s elements", name, len,
+                   field_name(f));
+        return false;
+    }
+    return true;
+}
+
+// ops.rs:309-316
+bool check_pair(int f, const char *name, const char *ln, size_t l, const char *rn, size_t r,
+                int *rc) {
+    if (l != r) {
+        *rc = fail("%s: %s is %zu bytes but %s is %zu bytes", name, ln, l, rn, r);
+        return false;
+    }
+    return check_width(f, name, l, rc);
+}
+
+bool checked_mul(size_t a, size_t b, size_t *out) { return !__builtin_mul_overflow(a, b, out); }
+
+// Host-side tables for a list of coefficients, entry i at i * entry_dw.
+std::vector<uint32_t> host_tables(int f, const uint64_t *vals, size_t n) {
+    std::vector<uint32_t> tab(n * entry_dw(field_bytes(f)));
+    for (size_t i = 0; i < n; ++i) build_coeff_table(f, vals[i], tab.data() + i * entry_dw(field_bytes(f)));
+    return tab;
+}
+
+void free_host_vector(void *p) { delete static_cast<std::vector<uint32_t> *>(p); }
+
+// Device tables for one call. Stream-ordered: the device buffer comes from
+// the stream's memory pool and is returned to it after the launches that
+// read it; the host staging vector is released by a host callback once the
+// copy has executed.
+struct TempTable {
+    uint32_t *d = nullptr;
+    hipStream_t s = nullptr;
+    ~TempTable() {
+        if (d) (void)hipFreeAsync(d, s);
+    }
+};
+
+int upload_temp(int f, const std::vector<uint64_t> &vals, hipStream_t s, TempTable &out) {
+    auto *host = new (std::nothrow) std::vector<uint32_t>(host_tables(f, vals.data(), vals.size()));
+    if (!host) return fail("out of host memory");
+    const size_t bytes = std::max<size_t>(host->size() * 4, 4);
+    hipError_t e = hipMallocAsync((void **)&out.d, bytes, s);
+    if (e != hipSuccess) {
+        delete host;
+        out.d = nullptr;
+        g_err = std::string("hipMallocAsync: ") + hipGetErrorString(e);
+        return FGF_ENOMEM;
+    }
+    out.s = s;
+    if (!host->empty()) {
+        e = hipMemcpyAsync(out.d, host->data(), host->size() * 4, hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) {
+            delete host;
+            return device_fail("hipMemcpyAsync", e);
+        }
+    }
+    e = hipLaunchHostFunc(s, free_host_vector, host);
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(s);
+        delete host;
+    }
+    return FGF_OK;
+}
+
+int run_matrix(int f, const uint32_t *tab, size_t stride_t, size_t stride_j, uint8_t *const *rows,
+               size_t m, const uint8_t *const *srcs, size_t k, size_t row_len, bool overwrite,
+               hipStream_t s) {
+    MatrixJob job;
+    job.nb = field_bytes(f);
+    job.srcs = srcs;
+    job.k = k;
+    job.rows = rows;
+    job.m = m;
+    job.tab = tab;
+    job.tab_stride_t = stride_t;
+    job.tab_stride_j = stride_j;
+    job.row_len = row_len;
+    job.overwrite = overwrite;
+    job.stream = s;
+    hipError_t e = launch_matrix(job);
+    return e == hipSuccess ? FGF_OK : device_fail("matrix launch", e);
+}
+
+int fill_zero(uint8_t *dst, size_t len, hipStream_t s) {
+    hipError_t e = launch_fill_zero(dst, len, s);
+    return e == hipSuccess ? FGF_OK : device_fail("fill launch", e);
+}
+
+// One-shot single-coefficient op: dst (^)= c * src over len bytes.
+int single(int f, uint8_t *dst, const uint8_t *src, size_t len, uint64_t c, bool overwrite,
+           hipStream_t s) {
+    if (len == 0) return FGF_OK;
+    TempTable tt;
+    int rc = upload_temp(f, {c}, s, tt);
+    if (rc) return rc;
+    return run_matrix(f, tt.d, 0, 0, &dst, 1, &src, 1, len, overwrite, s);
+}
+
+int single_plan(const fgf_plan *p, size_t index, uint8_t *dst, const uint8_t *src, size_t len,
+                bool overwrite, hipStream_t s) {
+    if (len == 0) return FGF_OK;
+    return run_matrix(p->field, p->d_tab + index * entry_dw(p->nb), 0, 0, &dst, 1, &src, 1, len,
+                      overwrite, s);
+}
+
+bool decode_coeffs(int f, const void *coeffs, size_t n, std::vector<uint64_t> &out, int *rc) {
+    const int nb = field_bytes(f);
+    out.resize(n);
+    const uint8_t *p = static_cast<const uint8_t *>(coeffs);
+    if (n && !p) {
+        *rc = fail("coefficient pointer is null");
+        return false;
+    }
+    for (size_t i = 0; i < n; ++i) out[i] = read_elem(p + i * nb, nb);
+    return true;
+}
+
+bool all_zero(const std::vector<uint64_t> &v) {
+    for (uint64_t x : v)
+        if (x) return false;
+    return true;
+}
+
+bool plan_ok(int f, const fgf_plan *p, const char *name, int *rc) {
+    if (!p) {
+        *rc = fail("%s: plan is null", name);
+        return false;
+    }
+    if (p->field != f) {
+        *rc = fail("%s: plan is over %s, not %s", name, field_name(p->field), field_name(f));
+        return false;
+    }
+    return true;
+}
+
+hipStream_t S(void *s) { return static_cast<hipStream_t>(s); }
+
+#define FIELD_GUARD(f)                                                  \
+    do {                                                                \
+        if (!field_ok(f)) return fail("unknown field id %d", (int)(f)); \
+    } while (0)
+
+// Validation shared by mul_add_matrix and dot_product_matrix (ops.rs:675-703,
+// 773-801): width, geometry, then per term its coefficient count and source.
+bool check_matrix_terms(int f, const char *name, size_t rows_len, size_t row_len, size_t nrows,
+                        const void *const *term_coeffs, const size_t *term_ncoeffs,
+                        const size_t *term_src_lens, size_t nterms, size_t *used, int *rc) {
+    if (!check_width(f, name, row_len, rc)) return false;
+    if (!checked_mul(nrows, row_len, used)) {
+        *rc = fail("%s: row geometry overflows", name);
+        return false;
+    }
+    if (rows_len < *used) {
+        *rc = fail("%s: rows is %zu bytes but %zu rows of %zu bytes need %zu", name, rows_len, nrows,
+                   row_len, *used);
+        return false;
+    }
+    for (size_t t = 0; t < nterms; ++t) {
+        if (term_ncoeffs[t] != nrows) {
+            *rc = fail("%s: term supplies %zu coefficients for %zu rows", name, term_ncoeffs[t], nrows);
+            return false;
+        }
+        if (term_src_lens[t] != row_len) {
+            *rc = fail("%s: source is %zu bytes, expected %zu", name, term_src_lens[t], row_len);
+            return false;
+        }
+    }
+    (void)term_coeffs;
+    return true;
+}
+
+// Flatten per-term coefficient slices into (t, j) at t*nrows + j.
+bool flatten_terms(int f, const void *const *term_coeffs, size_t nterms, size_t nrows,
+                   std::vector<uint64_t> &flat, int *rc) {
+    flat.resize(nterms * nrows);
+    std::vector<uint64_t> one;
+    for (size_t t = 0; t < nterms; ++t) {
+        if (!decode_coeffs(f, term_coeffs[t], nrows, one, rc)) return false;
+        std::copy(one.begin(), one.end(), flat.begin() + t * nrows);
+    }
+    return true;
+}
+
+// Validation shared by the _with matrix forms (ops.rs:727-748, 830-851).
+bool check_matrix_plan(int f, const char *name, size_t rows_len, size_t row_len, size_t nrows,
+                       const fgf_plan *plan, const size_t *src_lens, size_t nsrcs, size_t *used,
+                       int *rc) {
+    if (!check_width(f, name, row_len, rc)) return false;
+    if (!checked_mul(nrows, row_len, used)) {
+        *rc = fail("%s: row geometry overflows", name);
+        return false;
+    }
+    if (rows_len < *used) {
+        *rc = fail("%s: rows is %zu bytes but %zu rows of %zu bytes need %zu", name, rows_len, nrows,
+                   row_len, *used);
+        return false;
+    }
+    if (!plan_ok(f, plan, name, rc)) return false;
+    if (plan->rows != nsrcs || plan->cols != nrows) {
+        *rc = fail("%s: plan dimensions do not match (sources, rows)", name);
+        return false;
+    }
+    for (size_t i = 0; i < nsrcs; ++i) {
+        if (src_lens[i] != row_len) {
+            *rc = fail("%s: source %zu is %zu bytes, expected %zu", name, i, src_lens[i], row_len);
+            return false;
+        }
+    }
+    return true;
+}
+
+std::vector<uint8_t *> contiguous_rows(uint8_t *rows, size_t nrows, size_t row_len) {
+    std::vector<uint8_t *> r(nrows);
+    for (size_t j = 0; j < nrows; ++j) r[j] = rows + j * row_len;
+    return r;
+}
+
+int make_plan(int f, size_t rows, size_t cols, const std::vector<uint64_t> &vals, fgf_plan **out) {
+    for (uint64_t v : vals)
+        if (v & ~width_mask(f))
+            return fail("coefficient %#llx is not an element of %s", (unsigned long long)v,
+                        field_name(f));
+    auto *p = new (std::nothrow) fgf_plan;
+    if (!p) return fail("out of host memory");
+    p->field = f;
+    p->nb = field_bytes(f);
+    p->rows = rows;
+    p->cols = cols;
+    p->values = vals;
+    (void)hipGetDevice(&p->device);
+    std::vector<uint32_t> tab = host_tables(f, vals.data(), vals.size());
+    const size_t bytes = std::max<size_t>(tab.size() * 4, 4);
+    hipError_t e = hipMalloc((void **)&p->d_tab, bytes);
+    if (e != hipSuccess) {
+        delete p;
+        g_err = std::string("hipMalloc: ") + hipGetErrorString(e);
+        return FGF_ENOMEM;
+    }
+    if (!tab.empty() && (e = hipMemcpy(p->d_tab, tab.data(), tab.size() * 4, hipMemcpyHostToDevice)) != hipSuccess) {
+        (void)hipFree(p->d_tab);
+        delete p;
+        return device_fail("hipMemcpy", e);
+    }
+    *out = p;
+    return FGF_OK;
+}
+
+bool coeff_in_field(int f, uint64_t c, const char *name, int *rc) {
+    if (c & ~width_mask(f)) {
+        *rc = fail("%s: coefficient %#llx is not an element of %s", name, (unsigned long long)c,
+                   field_name(f));
+        return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *fgf_last_error(void) { return g_err.c_str(); }
+
+const char *fgf_backend(void) { return "hip-gfx950"; }
+
+int fgf_has_vector_elementwise(fgf_field field) { return field_ok(field) ? 1 : 0; }
+
+size_t fgf_field_bytes(fgf_field field) { return field_ok(field) ? (size_t)field_bytes(field) : 0; }
+
+const char *fgf_field_name(fgf_field field) { return field_name(field); }
+
+int fgf_elem_mul(fgf_field f, uint64_t a, uint64_t b, uint64_t *out) {
+    FIELD_GUARD(f);
+    int rc = 0;
+    if (!coeff_in_field(f, a, "elem_mul", &rc) || !coeff_in_field(f, b, "elem_mul", &rc)) return rc;
+    *out = HostField::mul(f, a, b);
+    return FGF_OK;
+}
+
+int fgf_elem_inv(fgf_field f, uint64_t a, uint64_t *out) {
+    FIELD_GUARD(f);
+    int rc = 0;
+    if (!coeff_in_field(f, a, "elem_inv", &rc)) return rc;
+    *out = HostField::inv(f, a);
+    return FGF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Plans (ops.rs:165-298).
+// ---------------------------------------------------------------------------
+
+int fgf_plan_new(fgf_field f, const void *coeffs, size_t n, fgf_plan **out) {
+    FIELD_GUARD(f);
+    if (!out) return fail("Plan::new: output pointer is null");
+    std::vector<uint64_t> vals;
+    int rc = 0;
+    if (!decode_coeffs(f, coeffs, n, vals, &rc)) return rc;
+    return make_plan(f, 1, n, vals, out);
+}
+
+int fgf_plan_matrix(fgf_field f, size_t rows, size_t cols, const void *coeffs, size_t n,
+                    fgf_plan **out) {
+    FIELD_GUARD(f);
+    if (!out) return fail("Plan::matrix: output pointer is null");
+    size_t len;
+    if (!checked_mul(rows, cols, &len)) return fail("Plan::matrix: dimensions overflow");
+    if (n != len) return fail("Plan::matrix: %zu coefficients for %zux%zu matrix", n, rows, cols);
+    std::vector<uint64_t> vals;
+    int rc = 0;
+    if (!decode_coeffs(f, coeffs, n, vals, &rc)) return rc;
+    return make_plan(f, rows, cols, vals, out);
+}
+
+int fgf_coeff_new(fgf_field f, uint64_t coeff, fgf_plan **out) {
+    FIELD_GUARD(f);
+    if (!out) return fail("Coeff::new: output pointer is null");
+    return make_plan(f, 1, 1, {coeff}, out);
+}
+
+void fgf_plan_free(fgf_plan *p) {
+    if (!p) return;
+    (void)hipFree(p->d_tab);
+    delete p;
+}
+
+size_t fgf_plan_len(const fgf_plan *p) { return p ? p->values.size() : 0; }
+
+void fgf_plan_dimensions(const fgf_plan *p, size_t *rows, size_t *cols) {
+    if (rows) *rows = p ? p->rows : 0;
+    if (cols) *cols = p ? p->cols : 0;
+}
+
+fgf_field fgf_plan_field(const fgf_plan *p) { return (fgf_field)(p ? p->field : -1); }
+
+int fgf_plan_get(const fgf_plan *p, size_t index, uint64_t *value) {
+    if (!p || index >= p->values.size()) return fail("Plan::get: index %zu out of bounds", index);
+    *value = p->values[index];
+    return FGF_OK;
+}
+
+int fgf_plan_get_at(const fgf_plan *p, size_t row, size_t col, uint64_t *value) {
+    if (!p || row >= p->rows || col >= p->cols)
+        return fail("Plan::get_at: (%zu, %zu) out of bounds", row, col);
+    *value = p->values[row * p->cols + col];
+    return FGF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Single-row operations (ops.rs:323-450).
+// ---------------------------------------------------------------------------
+
+int fgf_add_assign(fgf_field f, uint8_t *dst, size_t dst_len, const uint8_t *src, size_t src_len,
+                   void *stream) {
+    FIELD_GUARD(f);
+    int rc = 0;
+    if (!check_pair(f, "add_assign", "dst", dst_len, "src", src_len, &rc)) return rc;
+    return single(f, dst, src, dst_len, 1, false, S(stream));
+}
+
+int fgf_mul_add(fgf_field f, uint8_t *dst, size_t dst_len, uint64_t coeff, const uint8_t *src,
+                size_t src_len, void *stream) {
+    FIELD_GUARD(f);
+    int rc = 0;
+    if (!check_pair(f, "mul_add", "dst", dst_len, "src", src_len, &rc)) return rc;
+    if (!coeff_in_field(f, coeff, "mul_add", &rc)) return rc;
+    if (coeff == 0) return FGF_OK;
+    return single(f, dst, src, dst_len, coeff, false, S(stream));
+}
+
+int fgf_mul_add_with(fgf_field f, uint8_t *dst, size_t dst_len, const fgf_plan *plan, size_t index,
+                     const uint8_t *src, size_t src_len, void *stream) {
+    FIELD_GUARD(f);
+    int rc = 0;
+    if (!check_pair(f, "mul_add_with", "dst", dst_len, "src", src_len, &rc)) return rc;
+    if (!plan_ok(f, plan, "mul_add_with", &rc)) return rc;
+    if (index >= plan->values.size()) return fail("mul_add_with: coefficient index %zu out of bounds", index);
+    if (plan->values[index] == 0) return FGF_OK;
+    return single_plan(plan, index, dst, src, dst_len, false, S(stream));
+}
+
+int fgf_mul_into(fgf_field f, uint8_t *dst, size_t dst_len, uint64_t coeff, const uint8_t *src,
+                 size_t src_len, void *stream) {
+    FIELD_GUARD(f);
+    int rc = 0;
+    if (!check_pair(f, "mul_into", "dst", dst_len, "src", src_len, &rc)) return rc;
+    if (!coeff_in_field(f, coeff, "mul_into", &rc)) return rc;
+    if (coeff == 0) return fill_zero(dst, dst_len, S(stream));
+    return single(f, dst, src, dst_len, coeff, true, S(stream));
+}
+
+int fgf_mul_into_with(fgf_field f, uint8_t *dst, size_t dst_len, const fgf_plan *plan,
+                      size_t index, const uint8_t *src, size_t src_len, void *stream) {
+    FIELD_GUARD(f);
+    int rc = 0;
+    if (!check_pair(f, "mul_into_with", "dst", dst_len, "src", src_len, &rc)) return rc;
+    if (!plan_ok(f, plan, "mul_into_with", &rc)) return rc;
+    if (index >= plan->values.size()) return fail("mul_into_with: coefficient index %zu out of bounds", index);
+    if (plan->values[index] == 0) return fill_zero(dst, dst_len, S(stream));
+    return single_plan(plan, index, dst, src, dst_len, true, S(stream));
+}
+
+int fgf_mul_assign(fgf_field f, uint8_t *dst, size_t dst_len, uint64_t coeff, void *stream) {
+    FIELD_GUARD(f);
+    int rc = 0;
+    if (!check_width(f, "mul_assign", dst_len, &rc)) return rc;
+    if (!coeff_in_field(f, coeff, "mul_assign", &rc)) return rc;
+    if (coeff == 1) return FGF_OK;
+    if (coeff == 0) return fill_zero(dst, dst_len, S(stream));
+    return single(f, dst, dst, dst_len, coeff, true, S(stream));
+}
+
+int fgf_mul_assign_with(fgf_field f, uint8_t *dst, size_t dst_len, const fgf_plan *plan,
+                        size_t index, void *stream) {
+    FIELD_GUARD(f);
+    int rc = 0;
+    if (!check_width(f, "mul_assign_with", dst_len, &rc)) return rc;
+    if (!plan_ok(f, plan, "mul_assign_with", &rc)) return rc;
+    if (index >= plan->values.size()) return fail("mul_assign_with: coefficient index %zu out of bounds", index);
+    const uint64_t v = plan->values[index];
+    if (v == 1) return FGF_OK;
+    if (v == 0) return fill_zero(dst, dst_len, S(stream));
+    return single_plan(plan, index, dst, dst, dst_len, true, S(stream));
+}
+
+// ---------------------------------------------------------------------------
+// Scatter / gather / dot product (ops.rs:461-660).
+// ---------------------------------------------------------------------------
+
+int fgf_mul_add_scatter(fgf_field f, uint8_t *rows, size_t rows_len, size_t row_len,
+                        const void *coeffs, size_t ncoeffs, const uint8_t *src, size_t src_len,
+                        void *stream) {
+    FIELD_GUARD(f);
+    int rc = 0;
+    if (!check_pair(f, "mul_add_scatter", "row_len", row_len, "src", src_len, &rc)) return rc;
+    size_t used;
+    if (!checked_mul(ncoeffs, row_len, &used)) return fail("mul_add_scatter: row geometry overflows");
+    if (rows_len < used)
+        return fail("mul_add_scatter: rows is %zu bytes but %zu rows of %zu bytes need %zu", rows_len,
+                    ncoeffs, row_len, used);
+    std::vector<uint64_t> c;
+    if (!decode_coeffs(f, coeffs, ncoeffs, c, &rc)) return rc;
+    if (all_zero(c) || row_len == 0) return FGF_OK;
+    TempTable tt;
+    if ((rc = upload_temp(f, c, S(stream), tt))) return rc;
+    auto r = contiguous_rows(rows, ncoeffs, row_len);
+    return run_matrix(f, tt.d, 0, entry_dw(field_bytes(f)), r.data(), ncoeffs, &src, 1, row_len,
+                      false, S(stream));
+}
+
+int fgf_mul_add_scatter_with(fgf_field f, uint8_t *rows, size_t rows_len, size_t row_len,
+                             const fgf_plan *plan, const uint8_t *src, size_t src_len, void *stream) {
+    FIELD_GUARD(f);
+    int rc = 0;
+    if (!check_pair(f, "mul_add_scatter_with", "row_len", row_len, "src", src_len, &rc)) return rc;
+    if (!plan_ok(f, plan, "mul_add_scatter_with", &rc)) return rc;
+    const size_t n = plan->values.size();
+    size_t used;
+    if (!checked_mul(n, row_len, &used)) return fail("mul_add_scatter_with: row geometry overflows");
+    if (rows_len < used)
+        return fail("mul_add_scatter_with: rows is %zu bytes but %zu rows of %zu bytes need %zu",
+                    rows_len, n, row_len, used);
+    if (all_zero(plan->values) || row_len == 0) return FGF_OK;
+    auto r = contiguous_rows(rows, n, row_len);
+    return run_matrix(f, plan->d_tab, 0, entry_dw(plan->nb), r.data(), n, &src, 1, row_len, false,
+                      S(stream));
+}
+
+namespace {
+
+// mul_add_gather / dot_product: width, count, per-source length.
+bool check_gather(int f, const char *name, size_t dst_len, size_t ncoeffs, const size_t *src_lens,
+                  size_t nsrcs, bool indexed, bool plan_form, int *rc) {
+    if (!check_width(f, name, dst_len, rc)) return false;
+    if (ncoeffs != nsrcs) {
+        *rc = plan_form ? fail("%s: plan has %zu coefficients but there are %zu sources", name, ncoeffs, nsrcs)
+                        : fail("%s: %zu coefficients for %zu sources", name, ncoeffs, nsrcs);
+        return false;
+    }
+    for (size_t i = 0; i < nsrcs; ++i) {
+        if (src_lens[i] != dst_len) {
+            *rc = indexed ? fail("%s: source %zu is %zu bytes, expected %zu", name, i, src_lens[i], dst_len)
+                          : fail("%s: source is %zu bytes, expected %zu", name, src_lens[i], dst_len);
+            return false;
+        }
+    }
+    return true;
+}
+
+}  // namespace
+
+int fgf_mul_add_gather(fgf_field f, uint8_t *dst, size_t dst_len, const void *coeffs,
+                       size_t ncoeffs, const uint8_t *const *srcs, const size_t *src_lens,
+                       size_t nsrcs, void *stream) {
+    FIELD_GUARD(f);
+    int rc = 0;
+    if (!check_gather(f, "mul_add_gather", dst_len, ncoeffs, src_lens, nsrcs, false, false, &rc)) return rc;
+    std::vector<uint64_t> c;
+    if (!decode_coeffs(f, coeffs, ncoeffs, c, &rc)) return rc;
+    if (nsrcs == 0 || all_zero(c) || dst_len == 0) return FGF_OK;
+    TempTable tt;
+    if ((rc = upload_temp(f, c, S(stream), tt))) return rc;
+    return run_matrix(f, tt.d, entry_dw(field_bytes(f)), 0, &dst, 1, srcs, nsrcs, dst_len, false,
+                      S(stream));
+}
+
+int fgf_mul_add_gather_with(fgf_field f, uint8_t *dst, size_t dst_len, const fgf_plan *plan,
+                            const uint8_t *const *srcs, const size_t *src_lens, size_t nsrcs,
+                            void *stream) {
+    FIELD_GUARD(f);
+    int rc = 0;
+    if (!check_width(f, "mul_add_gather_with", dst_len, &rc)) return rc;
+    if (!plan_ok(f, plan, "mul_add_gather_with", &rc)) return rc;
+    if (!check_gather(f, "mul_add_gather_with", dst_len, plan->values.size(), src_lens, nsrcs, true,
+                      true, &rc))
+        return rc;
+    if (nsrcs == 0 || all_zero(plan->values) || dst_len == 0) return FGF_OK;
+    return run_matrix(f, plan->d_tab, entry_dw(plan->nb), 0, &dst, 1, srcs, nsrcs, dst_len, false,
+                      S(stream));
+}
+
+int fgf_dot_product(fgf_field f, uint8_t *dst, size_t dst_len, const void *coeffs, size_t ncoeffs,
+                    const uint8_t *const *srcs, const size_t *src_lens, size_t nsrcs, void *stream) {
+    FIELD_GUARD(f);
+    int rc = 0;
+    if (!check_gather(f, "dot_product", dst_len, ncoeffs, src_lens, nsrcs, true, false, &rc)) return rc;
+    std::vector<uint64_t> c;
+    if (!decode_coeffs(f, coeffs, ncoeffs, c, &rc)) return rc;
+    if (nsrcs == 0 || all_zero(c)) return fill_zero(dst, dst_len, S(stream));
+    if (ncoeffs == 1) return fgf_mul_into(f, dst, dst_len, c[0], srcs[0], src_lens[0], stream);
+    if (dst_len == 0) return FGF_OK;
+    TempTable tt;
+    if ((rc = upload_temp(f, c, S(stream), tt))) return rc;
+    return run_matrix(f, tt.d, entry_dw(field_bytes(f)), 0, &dst, 1, srcs, nsrcs, dst_len, true,
+                      S(stream));
+}
+
+int fgf_dot_product_with(fgf_field f, uint8_t *dst, size_t dst_len, const fgf_plan *plan,
+                         const uint8_t *const *srcs, const size_t *src_lens, size_t nsrcs,
+                         void *stream) {
+    FIELD_GUARD(f);
+    int rc = 0;
+    if (!check_width(f, "dot_product_with", dst_len, &rc)) return rc;
+    if (!plan_ok(f, plan, "dot_product_with", &rc)) return rc;
+    if (!check_gather(f, "dot_product_with", dst_len, plan->values.size(), src_lens, nsrcs, true, true,
+                      &rc))
+        return rc;
+    if (nsrcs == 0 || all_zero(plan->values)) return fill_zero(dst, dst_len, S(stream));
+    if (plan->values.size() == 1)
+        return fgf_mul_into_with(f, dst, dst_len, plan, 0, srcs[0], src_lens[0], stream);
+    if (dst_len == 0) return FGF_OK;
+    return run_matrix(f, plan->d_tab, entry_dw(plan->nb), 0, &dst, 1, srcs, nsrcs, dst_len, true,
+                      S(stream));
+}
+
+// ---------------------------------------------------------------------------
+// Matrix shapes (ops.rs:675-941).
+// ---------------------------------------------------------------------------
+
+int fgf_mul_add_matrix(fgf_field f, uint8_t *rows, size_t rows_len, size_t row_len, size_t nrows,
+                       const void *const *term_coeffs, const size_t *term_ncoeffs,
+                       const uint8_t *const *term_srcs, const size_t *term_src_lens, size_t nterms,
+                       void *stream) {
+    FIELD_GUARD(f);
+    int rc = 0;
+    size_t used;
+    if (!check_matrix_terms(f, "mul_add_matrix", rows_len, row_len, nrows, term_coeffs, term_ncoeffs,
+                            term_src_lens, nterms, &used, &rc))
+        return rc;
+    if (nrows == 0 || nterms == 0 || row_len == 0) return FGF_OK;
+    std::vector<uint64_t> flat;
+    if (!flatten_terms(f, term_coeffs, nterms, nrows, flat, &rc)) return rc;
+    TempTable tt;
+    if ((rc = upload_temp(f, flat, S(stream), tt))) return rc;
+    const size_t e = entry_dw(field_bytes(f));
+    auto r = contiguous_rows(rows, nrows, row_len);
+    return run_matrix(f, tt.d, nrows * e, e, r.data(), nrows, term_srcs, nterms, row_len, false,
+                      S(stream));
+}
+
+int fgf_mul_add_matrix_with(fgf_field f, uint8_t *rows, size_t rows_len, size_t row_len,
+                            size_t nrows, const fgf_plan *plan, const uint8_t *const *srcs,
+                            const size_t *src_lens, size_t nsrcs, void *stream) {
+    FIELD_GUARD(f);
+    int rc = 0;
+    size_t used;
+    if (!check_matrix_plan(f, "mul_add_matrix_with", rows_len, row_len, nrows, plan, src_lens, nsrcs,
+                           &used, &rc))
+        return rc;
+    if (nrows == 0 || nsrcs == 0 || row_len == 0) return FGF_OK;
+    const size_t e = entry_dw(plan->nb);
+    auto r = contiguous_rows(rows, nrows, row_len);
+    return run_matrix(f, plan->d_tab, nrows * e, e, r.data(), nrows, srcs, nsrcs, row_len, false,
+                      S(stream));
+}
+
+int fgf_dot_product_matrix(fgf_field f, uint8_t *rows, size_t rows_len, size_t row_len,
+                           size_t nrows, const void *const *term_coeffs, const size_t *term_ncoeffs,
+                           const uint8_t *const *term_srcs, const size_t *term_src_lens,
+                           size_t nterms, void *stream) {
+    FIELD_GUARD(f);
+    int rc = 0;
+    size_t used;
+    if (!check_matrix_terms(f, "dot_product_matrix", rows_len, row_len, nrows, term_coeffs,
+                            term_ncoeffs, term_src_lens, nterms, &used, &rc))
+        return rc;
+    if (nrows == 0) return FGF_OK;
+    if (nterms == 0) return fill_zero(rows, used, S(stream));
+    if (row_len == 0) return FGF_OK;
+    std::vector<uint64_t> flat;
+    if (!flatten_terms(f, term_coeffs, nterms, nrows, flat, &rc)) return rc;
+    TempTable tt;
+    if ((rc = upload_temp(f, flat, S(stream), tt))) return rc;
+    const size_t e = entry_dw(field_bytes(f));
+    auto r = contiguous_rows(rows, nrows, row_len);
+    return run_matrix(f, tt.d, nrows * e, e, r.data(), nrows, term_srcs, nterms, row_len, true,
+                      S(stream));
+}
+
+int fgf_dot_product_matrix_with(fgf_field f, uint8_t *rows, size_t rows_len, size_t row_len,
+                                size_t nrows, const fgf_plan *plan, const uint8_t *const *srcs,
+                                const size_t *src_lens, size_t nsrcs, void *stream) {
+    FIELD_GUARD(f);
+    int rc = 0;
+    size_t used;
+    if (!check_matrix_plan(f, "dot_product_matrix_with", rows_len, row_len, nrows, plan, src_lens,
+                           nsrcs, &used, &rc))
+        return rc;
+    if (nrows == 0) return FGF_OK;
+    if (nsrcs == 0) return fill_zero(rows, used, S(stream));
+    if (row_len == 0) return FGF_OK;
+    const size_t e = entry_dw(plan->nb);
+    auto r = contiguous_rows(rows, nrows, row_len);
+    return run_matrix(f, plan->d_tab, nrows * e, e, r.data(), nrows, srcs, nsrcs, row_len, true,
+                      S(stream));
+}
+
+namespace {
+
+// ops.rs:893-921: width, bounds and alignment per row, then disjointness.
+bool check_scattered_rows(int f, const char *name, size_t dst_len, size_t row_len,
+                          const size_t *row_starts, size_t nrows, int *rc) {
+    if (!check_width(f, name, row_len, rc)) return false;
+    const size_t nb = (size_t)field_bytes(f);
+    for (size_t j = 0; j < nrows; ++j) {
+        const size_t start = row_starts[j];
+        size_t end;
+        if (__builtin_add_overflow(start, row_len, &end)) {
+            *rc = fail("%s: row offset + length overflows", name);
+            return false;
+        }
+        if (end > dst_len) {
+            *rc = fail("%s: row %zu spans %zu..%zu but dst is %zu bytes", name, j, start, end, dst_len);
+            return false;
+        }
+        if (start % nb != 0) {
+            *rc = fail("%s: row %zu offset %zu is not a whole number of %s elements", name, j, start,
+                       field_name(f));
+            return false;
+        }
+    }
+    for (size_t a = 0; a < nrows; ++a) {
+        for (size_t b = a + 1; b < nrows; ++b) {
+            const size_t sa = row_starts[a], sb = row_starts[b];
+            const size_t lo = sa <= sb ? sa : sb, hi = sa <= sb ? sb : sa;
+            if (hi - lo < row_len) {
+                *rc = fail("%s: rows at %zu and %zu overlap for %zu-byte rows", name, sa, sb, row_len);
+                return false;
+            }
+        }
+    }
+    return true;
+}
+
+}  // namespace
+
+int fgf_mul_add_matrix_scattered(fgf_field f, uint8_t *dst, size_t dst_len, size_t row_len,
+                                 const size_t *row_starts, size_t nrows,
+                                 const void *const *term_coeffs, const size_t *term_ncoeffs,
+                                 const uint8_t *const *term_srcs, const size_t *term_src_lens,
+                                 size_t nterms, void *stream) {
+    FIELD_GUARD(f);
+    const char *name = "mul_add_matrix_scattered";
+    int rc = 0;
+    if (!check_scattered_rows(f, name, dst_len, row_len, row_starts, nrows, &rc)) return rc;
+    for (size_t t = 0; t < nterms; ++t) {
+        if (term_ncoeffs[t] != nrows)
+            return fail("%s: term supplies %zu coefficients for %zu rows", name, term_ncoeffs[t], nrows);
+        if (term_src_lens[t] != row_len)
+            return fail("%s: source is %zu bytes, expected %zu", name, term_src_lens[t], row_len);
+    }
+    if (nrows == 0 || nterms == 0 || row_len == 0) return FGF_OK;
+    std::vector<uint64_t> flat;
+    if (!flatten_terms(f, term_coeffs, nterms, nrows, flat, &rc)) return rc;
+    TempTable tt;
+    if ((rc = upload_temp(f, flat, S(stream), tt))) return rc;
+    std::vector<uint8_t *> r(nrows);
+    for (size_t j = 0; j < nrows; ++j) r[j] = dst + row_starts[j];
+    const size_t e = entry_dw(field_bytes(f));
+    return run_matrix(f, tt.d, nrows * e, e, r.data(), nrows, term_srcs, nterms, row_len, false,
+                      S(stream));
+}
+
+int fgf_mul_add_matrix_scattered_with(fgf_field f, uint8_t *dst, size_t dst_len, size_t row_len,
+                                      const size_t *row_starts, size_t nrows, const fgf_plan *plan,
+                                      const uint8_t *const *srcs, const size_t *src_lens,
+                                      size_t nsrcs, void *stream) {
+    FIELD_GUARD(f);
+    const char *name = "mul_add_matrix_scattered_with";
+    int rc = 0;
+    if (!check_scattered_rows(f, name, dst_len, row_len, row_starts, nrows, &rc)) return rc;
+    if (!plan_ok(f, plan, name, &rc)) return rc;
+    if (plan->rows != nsrcs || plan->cols != nrows)
+        return fail("%s: plan dimensions do not match (sources, rows)", name);
+    for (size_t i = 0; i < nsrcs; ++i)
+        if (src_lens[i] != row_len)
+            return fail("%s: source %zu is %zu bytes, expected %zu", name, i, src_lens[i], row_len);
+    if (nrows == 0 || nsrcs == 0 || row_len == 0) return FGF_OK;
+    std::vector<uint8_t *> r(nrows);
+    for (size_t j = 0; j < nrows; ++j) r[j] = dst + row_starts[j];
+    const size_t e = entry_dw(plan->nb);
+    return run_matrix(f, plan->d_tab, nrows * e, e, r.data(), nrows, srcs, nsrcs, row_len, false,
+                      S(stream));
+}
+
+int fgf_mul_elementwise(fgf_field f, uint8_t *dst, size_t dst_len, const uint8_t *a, size_t a_len,
+                        const uint8_t *b, size_t b_len, void *stream) {
+    FIELD_GUARD(f);
+    int rc = 0;
+    if (!check_pair(f, "mul_elementwise", "dst", dst_len, "a", a_len, &rc)) return rc;
+    if (!check_pair(f, "mul_elementwise", "dst", dst_len, "b", b_len, &rc)) return rc;
+    hipError_t e = launch_elementwise(f, dst, a, b, dst_len, S(stream));
+    return e == hipSuccess ? FGF_OK : device_fail("elementwise launch", e);
+}
+
+int fgf_coeff_table(fgf_field f, uint64_t coeff, uint32_t *out) {
+    FIELD_GUARD(f);
+    int rc = 0;
+    if (!coeff_in_field(f, coeff, "coeff_table", &rc)) return rc;
+    build_coeff_table(f, coeff, out);
+    return FGF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Packing (ops.rs:963-1001).
+// ---------------------------------------------------------------------------
+
+int fgf_pack(fgf_field f, uint8_t *dst, size_t dst_len, const uint64_t *elems, size_t n) {
+    FIELD_GUARD(f);
+    const int nb = field_bytes(f);
+    size_t expected;
+    if (!checked_mul(n, (size_t)nb, &expected)) return fail("pack: buffer geometry overflows");
+    if (dst_len != expected)
+        return fail("pack: dst is %zu bytes but %zu %s elements need %zu", dst_len, n, field_name(f),
+                    expected);
+    for (size_t i = 0; i < n; ++i) {
+        if (elems[i] & ~width_mask(f))
+            return fail("pack: element %#llx is not an element of %s", (unsigned long long)elems[i],
+                        field_name(f));
+    }
+    for (size_t i = 0; i < n; ++i)
+        for (int k = 0; k < nb; ++k) dst[i * nb + k] = (uint8_t)(elems[i] >> (8 * k));
+    return FGF_OK;
+}
+
+int fgf_unpack(fgf_field f, uint64_t *dst, size_t n, const uint8_t *src, size_t src_len) {
+    FIELD_GUARD(f);
+    const int nb = field_bytes(f);
+    size_t expected;
+    if (!checked_mul(n, (size_t)nb, &expected)) return fail("unpack: buffer geometry overflows");
+    if (src_len != expected)
+        return fail("unpack: src is %zu bytes but dst holds %zu %s elements (%zu bytes)", src_len, n,
+                    field_name(f), expected);
+    for (size_t i = 0; i < n; ++i) dst[i] = read_elem(src + i * nb, nb);
+    return FGF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,194 @@
+// Field definitions shared by the host plan builder and the device kernels.
+//
+// Nine binary fields, all multiplied through one byte subfield K:
+//   Gf8B  GF(2)[x]/0x11B, generator 0x03        (src/field/gf8b.rs:97)
+//   Gf8D  GF(2)[x]/0x11D, generator 0x02        (src/field/gf8d.rs:40)
+//   Gf16/32/64  Karatsuba towers over Gf8B, u^2 = u + DELTA with DELTA =
+//         0x20, 0x2000, 0x2000_0000             (gf16.rs:55,143-153; gf32.rs:37,121-129;
+//                                                gf64.rs:36,120-128)
+//   FanPaar8/16/32/64  canonical towers X^2 + alpha X + 1 over fp8, byte
+//         generator 0x2d                        (src/field/fan_paar.rs:58-174)
+// Elements are packed little endian (src/field/mod.rs:9-15): byte q of an
+// element is its coordinate on the q-th K-basis vector, so multiplying by a
+// fixed c is a B x B matrix over K acting on the element's bytes.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+#ifndef FGF_HD
+#if defined(__HIPCC__)
+#define FGF_HD __host__ __device__
+#else
+#define FGF_HD
+#endif
+#endif
+
+namespace fgf {
+
+enum FieldId : int {
+    GF8B = 0, GF8D = 1, GF16 = 2, GF32 = 3, GF64 = 4,
+    FP8 = 5, FP16 = 6, FP32 = 7, FP64 = 8, FIELD_COUNT = 9
+};
+
+FGF_HD constexpr bool valid_field(int f) { return f >= 0 && f < FIELD_COUNT; }
+FGF_HD constexpr bool is_fan_paar(int f) { return f >= FP8; }
+
+FGF_HD constexpr int field_bytes(int f) {
+    return (f == GF8B || f == GF8D || f == FP8) ? 1
+         : (f == GF16 || f == FP16)             ? 2
+         : (f == GF32 || f == FP32)             ? 4
+                                                : 8;
+}
+
+// The byte subfield every product reduces to.
+FGF_HD constexpr int byte_field(int f) { return f == GF8D ? GF8D : is_fan_paar(f) ? FP8 : GF8B; }
+
+// Field::NAME of the reference (flat8.rs:248; gf16.rs:249; gf32.rs:221;
+// gf64.rs:220; fan_paar.rs:403 with the names at 536-590).
+inline const char *field_name(int f) {
+    static const char *const names[FIELD_COUNT] = {
+        "GF(2^8)", "GF(2^8)/0x11D", "GF(2^16)", "GF(2^32)", "GF(2^64)",
+        "Fan-Paar GF(2^8)", "Fan-Paar GF(2^16)", "Fan-Paar GF(2^32)", "Fan-Paar GF(2^64)"};
+    return valid_field(f) ? names[f] : "?";
+}
+
+// ---------------------------------------------------------------------------
+// Bit-level definitions (used to derive the log/exp tables).
+// ---------------------------------------------------------------------------
+
+// Polynomial-basis product modulo x^8 + low (low = 0x1B or 0x1D).
+FGF_HD constexpr uint8_t poly_mul8(uint8_t a, uint8_t b, uint8_t low) {
+    uint8_t r = 0;
+    for (int i = 0; i < 8; ++i) {
+        r = (uint8_t)(r ^ ((b >> i) & 1 ? a : 0));
+        a = (uint8_t)((a << 1) ^ ((a & 0x80) ? low : 0));
+    }
+    return r;
+}
+
+FGF_HD constexpr uint64_t lowbits(unsigned bits) { return bits >= 64 ? ~0ull : ((1ull << bits) - 1); }
+
+// Multiplication by the level's tower generator, recursively (fan_paar.rs:68-77).
+FGF_HD constexpr uint64_t fp_alpha(uint64_t v, unsigned bits) {
+    if (bits == 1) return v & 1;
+    unsigned h = bits / 2;
+    uint64_t lo = v & lowbits(h), hi = v >> h;
+    return hi | ((lo ^ fp_alpha(hi, h)) << h);
+}
+
+// Canonical tower product down to single bits (fan_paar.rs:80-94).
+FGF_HD constexpr uint64_t fp_mul_bits(uint64_t x, uint64_t y, unsigned bits) {
+    if (bits == 1) return x & y & 1;
+    unsigned h = bits / 2;
+    uint64_t m = lowbits(h);
+    uint64_t x0 = x & m, x1 = x >> h, y0 = y & m, y1 = y >> h;
+    uint64_t lo = fp_mul_bits(x0, y0, h), hi = fp_mul_bits(x1, y1, h);
+    uint64_t mid = fp_mul_bits(x0 ^ x1, y0 ^ y1, h) ^ lo ^ hi;
+    return (lo ^ hi) | ((mid ^ fp_alpha(hi, h)) << h);
+}
+
+// Discrete-log tables of one byte field: exp doubled to 510 entries so a sum
+// of two logs indexes it without a modulo.
+struct ByteLogTables {
+    uint8_t exp[512];
+    uint8_t log[256];
+};
+
+FGF_HD constexpr uint8_t byte_gen_step(int bf, uint8_t v) {
+    return bf == GF8D ? poly_mul8(v, 0x02, 0x1D)
+         : bf == GF8B ? poly_mul8(v, 0x03, 0x1B)
+                      : (uint8_t)fp_mul_bits(v, 0x2D, 8);
+}
+
+FGF_HD constexpr ByteLogTables make_log_tables(int bf) {
+    ByteLogTables t{};
+    uint8_t v = 1;
+    for (int i = 0; i < 255; ++i) {
+        t.exp[i] = v;
+        t.exp[i + 255] = v;
+        t.log[v] = (uint8_t)i;
+        v = byte_gen_step(bf, v);
+    }
+    t.exp[510] = t.exp[0];
+    t.exp[511] = t.exp[1];
+    return t;
+}
+
+// ---------------------------------------------------------------------------
+// Host arithmetic over any field, via the byte field's log/exp tables.
+// ---------------------------------------------------------------------------
+
+struct HostField {
+    static const ByteLogTables &tables(int bf) {
+        static const ByteLogTables t8b = make_log_tables(GF8B);
+        static const ByteLogTables t8d = make_log_tables(GF8D);
+        static const ByteLogTables tfp = make_log_tables(FP8);
+        return bf == GF8D ? t8d : bf == FP8 ? tfp : t8b;
+    }
+
+    static uint8_t bmul(int bf, uint8_t a, uint8_t b) {
+        if (!a || !b) return 0;
+        const ByteLogTables &t = tables(bf);
+        return t.exp[t.log[a] + t.log[b]];
+    }
+
+    static uint64_t tower_delta(unsigned bits) {
+        return bits == 16 ? 0x20u : bits == 32 ? 0x2000u : 0x20000000ull;
+    }
+
+    // Karatsuba over Gf8B with the DELTA reduction of each level.
+    static uint64_t gmul(uint64_t x, uint64_t y, unsigned bits) {
+        if (bits == 8) return bmul(GF8B, (uint8_t)x, (uint8_t)y);
+        unsigned h = bits / 2;
+        uint64_t m = lowbits(h);
+        uint64_t x0 = x & m, x1 = x >> h, y0 = y & m, y1 = y >> h;
+        uint64_t p00 = gmul(x0, y0, h), p11 = gmul(x1, y1, h);
+        uint64_t lo = p00 ^ gmul(tower_delta(bits), p11, h);
+        uint64_t hi = gmul(x0 ^ x1, y0 ^ y1, h) ^ p00;
+        return lo | (hi << h);
+    }
+
+    static uint64_t fmul(uint64_t x, uint64_t y, unsigned bits) {
+        if (bits == 8) return bmul(FP8, (uint8_t)x, (uint8_t)y);
+        unsigned h = bits / 2;
+        uint64_t m = lowbits(h);
+        uint64_t x0 = x & m, x1 = x >> h, y0 = y & m, y1 = y >> h;
+        uint64_t lo = fmul(x0, y0, h), hi = fmul(x1, y1, h);
+        uint64_t mid = fmul(x0 ^ x1, y0 ^ y1, h) ^ lo ^ hi;
+        return (lo ^ hi) | ((mid ^ fp_alpha(hi, h)) << h);
+    }
+
+    static uint64_t mul(int f, uint64_t a, uint64_t b) {
+        unsigned bits = 8u * (unsigned)field_bytes(f);
+        if (f == GF8B || f == GF8D) return bmul(f, (uint8_t)a, (uint8_t)b);
+        if (is_fan_paar(f)) return fmul(a, b, bits);
+        return gmul(a, b, bits);
+    }
+
+    // a^(2^bits - 2): the inverse in any of these fields (zero maps to zero).
+    static uint64_t inv(int f, uint64_t a) {
+        unsigned bits = 8u * (unsigned)field_bytes(f);
+        if (a == 0) return 0;
+        uint64_t e = (bits == 64 ? ~0ull : ((1ull << bits) - 1)) - 1;
+        uint64_t r = 1, base = a;
+        while (e) {
+            if (e & 1) r = mul(f, r, base);
+            base = mul(f, base, base);
+            e >>= 1;
+        }
+        return r;
+    }
+
+    // M[p][q] = byte p of c * e_q, where e_q has byte q equal to one: the
+    // K-matrix of multiplication by c in byte coordinates.
+    static void byte_matrix(int f, uint64_t c, uint8_t *m /* B*B row-major */) {
+        int nb = field_bytes(f);
+        for (int q = 0; q < nb; ++q) {
+            uint64_t col = mul(f, c, 1ull << (8 * q));
+            for (int p = 0; p < nb; ++p) m[p * nb + q] = (uint8_t)(col >> (8 * p));
+        }
+    }
+};
+
+}  // namespace fgf

@@ -1,0 +1,614 @@
+// HIP kernels for fgf's packed vector operations on gfx950.
+//
+// Matrix family (dot_product_matrix, mul_add_matrix, mul_add_matrix_scattered,
+// scatter, gather, dot_product, mul_add, mul_into, mul_assign, add_assign):
+//   rows[j] (=|^=) sum_t c(t, j) * src[t]
+// Multiplying a packed element by a fixed coefficient is a B x B matrix over
+// the byte subfield (src/kernel/tables.rs:223-237 states the same fact for the
+// GF(2^16) tower). Each of its byte constants l is applied to four bytes at a
+// time with three v_perm_b32 lookups on 3-, 3- and 2-bit slices of every byte.
+// Elements of width B are transposed in registers into B byte planes of four
+// elements each, so every plane is multiplied by one constant per (p, q).
+// A wave keeps the M destination rows of its column tile in VGPRs while it
+// folds in all k sources, so every source byte crosses HBM once and every
+// destination byte is written once (the blocking of the reference's
+// x86 matrix_impl, src/kernel/x86/gf8.rs:2024-2067, at GPU tile sizes).
+//
+// Elementwise family (mul_elementwise): products through discrete-log tables
+// of the byte field held in LDS, composed by the tower recursions.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "field.hpp"
+#include "kernels.hpp"
+
+namespace fgf {
+
+// ---------------------------------------------------------------------------
+// Host: table construction.
+// ---------------------------------------------------------------------------
+
+void build_coeff_table(int field, uint64_t c, uint32_t *out) {
+    const int nb = field_bytes(field);
+    const int bf = byte_field(field);
+    uint8_t m[64];
+    HostField::byte_matrix(field, c, m);
+    for (int pq = 0; pq < nb * nb; ++pq) {
+        const uint8_t l = m[pq];
+        uint8_t t0[8], t1[8], t2[4];
+        for (int i = 0; i < 8; ++i) {
+            t0[i] = HostField::bmul(bf, l, (uint8_t)i);
+            t1[i] = HostField::bmul(bf, l, (uint8_t)(i << 3));
+        }
+        for (int i = 0; i < 4; ++i) t2[i] = HostField::bmul(bf, l, (uint8_t)(i << 6));
+        uint32_t *e = out + pq * TAB_DW;
+        std::memcpy(&e[0], t0, 8);
+        std::memcpy(&e[2], t1, 8);
+        std::memcpy(&e[4], t2, 4);
+    }
+}
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Device building blocks.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t vperm(uint32_t hi, uint32_t lo, uint32_t sel) {
+    return __builtin_amdgcn_perm(hi, lo, sel);
+}
+
+// Byte-plane transforms for a group of four elements held in B dwords.
+// Input: the group's bytes in memory order (element e at bytes e*B..e*B+B-1,
+// for B = 8 element e is dwords 2e, 2e+1). Output: plane p holds byte p of
+// elements 0..3.
+__device__ __forceinline__ void transpose4(uint32_t &a, uint32_t &b, uint32_t &c, uint32_t &d) {
+    const uint32_t t0 = vperm(b, a, 0x05010400u), t1 = vperm(b, a, 0x07030602u);
+    const uint32_t t2 = vperm(d, c, 0x05010400u), t3 = vperm(d, c, 0x07030602u);
+    a = vperm(t2, t0, 0x05040100u);
+    b = vperm(t2, t0, 0x07060302u);
+    c = vperm(t3, t1, 0x05040100u);
+    d = vperm(t3, t1, 0x07060302u);
+}
+
+template <int B>
+__device__ __forceinline__ void to_planes(const uint32_t *w, uint32_t *p) {
+    if constexpr (B == 1) {
+        p[0] = w[0];
+    } else if constexpr (B == 2) {
+        p[0] = vperm(w[1], w[0], 0x06040200u);
+        p[1] = vperm(w[1], w[0], 0x07050301u);
+    } else if constexpr (B == 4) {
+        p[0] = w[0]; p[1] = w[1]; p[2] = w[2]; p[3] = w[3];
+        transpose4(p[0], p[1], p[2], p[3]);
+    } else {
+        p[0] = w[0]; p[1] = w[2]; p[2] = w[4]; p[3] = w[6];
+        p[4] = w[1]; p[5] = w[3]; p[6] = w[5]; p[7] = w[7];
+        transpose4(p[0], p[1], p[2], p[3]);
+        transpose4(p[4], p[5], p[6], p[7]);
+    }
+}
+
+template <int B>
+__device__ __forceinline__ void from_planes(const uint32_t *p, uint32_t *w) {
+    if constexpr (B == 1) {
+        w[0] = p[0];
+    } else if constexpr (B == 2) {
+        w[0] = vperm(p[1], p[0], 0x05010400u);
+        w[1] = vperm(p[1], p[0], 0x07030602u);
+    } else if constexpr (B == 4) {
+        w[0] = p[0]; w[1] = p[1]; w[2] = p[2]; w[3] = p[3];
+        transpose4(w[0], w[1], w[2], w[3]);
+    } else {
+        uint32_t a = p[0], b = p[1], c = p[2], d = p[3];
+        transpose4(a, b, c, d);
+        w[0] = a; w[2] = b; w[4] = c; w[6] = d;
+        a = p[4]; b = p[5]; c = p[6]; d = p[7];
+        transpose4(a, b, c, d);
+        w[1] = a; w[3] = b; w[5] = c; w[7] = d;
+    }
+}
+
+struct Sel {
+    uint32_t s0, s1, s2;
+};
+
+__device__ __forceinline__ Sel selectors(uint32_t v) {
+    return {v & 0x07070707u, (v >> 3) & 0x07070707u, (v >> 6) & 0x03030303u};
+}
+
+// a ^ b ^ c in one v_bitop3_b32 (truth table 0x96).
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
+// acc ^ l * v for the byte constant l whose tables start at e: three lookups
+// folded into the accumulator with two ops.
+__device__ __forceinline__ uint32_t cmul_acc(uint32_t acc, const uint32_t *__restrict__ e,
+                                             const Sel &s) {
+    return xor3(acc, vperm(e[1], e[0], s.s0), vperm(e[3], e[2], s.s1)) ^ vperm(e[4], e[4], s.s2);
+}
+
+struct MatrixArgs {
+    const uint8_t *src[KMAX];
+    uint8_t *dst[MMAX];
+    const uint32_t *tab;     // entry (t = 0, j = 0) of this launch
+    uint64_t col0;           // first byte column of the segment
+    uint64_t ncols;          // bytes in the segment
+    uint32_t k;              // sources in this launch
+    uint32_t m;              // rows in this launch
+    uint32_t tab_stride_t;   // dwords
+    uint32_t tab_stride_j;   // dwords
+    uint32_t ngroups;        // row groups of M rows
+    uint32_t ncoltiles;      // column tiles (vector kernel) or groups of 4 elements (generic)
+};
+
+constexpr int BLOCK = 256;
+
+// Block index -> (column tile, row group). Blocks are dealt to the XCDs round
+// robin, so with several row groups the blocks b, b + 8, b + 16, ... (one XCD)
+// visit every row group of one column tile before moving on: the tile's
+// source bytes are re-read from that XCD's L2, not from HBM.
+__device__ __forceinline__ bool map_block(const MatrixArgs &a, uint32_t &ct, uint32_t &rg) {
+    const uint32_t b = blockIdx.x;
+    if (a.ngroups == 1) {
+        ct = b;
+        rg = 0;
+    } else {
+        const uint32_t lo = b & 7u, rest = b >> 3;
+        rg = rest % a.ngroups;
+        ct = (rest / a.ngroups) * 8u + lo;
+    }
+    return ct < a.ncoltiles;
+}
+
+// Vector kernel: every pointer + col0 is 16-byte aligned and ncols is a
+// multiple of 16. A lane owns D dwords per source: 16-byte pieces at
+// tile + i*1024 + lane*16, so each load instruction covers 1 KiB contiguous.
+template <int B, int M, int D, bool OW>
+__global__ __launch_bounds__(BLOCK) void k_matrix(const MatrixArgs a) {
+    constexpr int P = D / 4;             // 16-byte pieces per lane
+    constexpr int G = D / B;             // groups of four elements per lane
+    constexpr uint32_t WAVE_BYTES = 256u * D;
+    static_assert(D % 4 == 0 && D % B == 0, "tile shape");
+
+    uint32_t ct, rg;
+    if (!map_block(a, ct, rg)) return;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint64_t tile = ((uint64_t)ct * (BLOCK / 64) + wave) * WAVE_BYTES;
+    if (tile >= a.ncols) return;
+    const uint32_t j0 = rg * M;
+    const uint32_t mrows = min((uint32_t)M, a.m - j0);
+
+    uint64_t off[P];
+    bool ok[P];
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+        const uint64_t rel = tile + (uint64_t)i * 1024u + lane * 16u;
+        ok[i] = rel < a.ncols;
+        off[i] = a.col0 + rel;
+    }
+
+    uint32_t acc[M][D];
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        uint32_t w[D];
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (!OW && j < (int)mrows && ok[i]) v = *(const uint4 *)(a.dst[j0 + j] + off[i]);
+            w[4 * i + 0] = v.x; w[4 * i + 1] = v.y; w[4 * i + 2] = v.z; w[4 * i + 3] = v.w;
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) to_planes<B>(&w[g * B], &acc[j][g * B]);
+    }
+
+    uint4 cur[P], nxt[P];
+#pragma unroll
+    for (int i = 0; i < P; ++i)
+        cur[i] = ok[i] ? *(const uint4 *)(a.src[0] + off[i]) : make_uint4(0, 0, 0, 0);
+
+    const uint32_t *tab_rg = a.tab + (size_t)j0 * a.tab_stride_j;
+    for (uint32_t t = 0; t < a.k; ++t) {
+        if (t + 1 < a.k) {
+            const uint8_t *s = a.src[t + 1];
+#pragma unroll
+            for (int i = 0; i < P; ++i)
+                nxt[i] = ok[i] ? *(const uint4 *)(s + off[i]) : make_uint4(0, 0, 0, 0);
+        }
+        uint32_t w[D], pl[D];
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+            w[4 * i + 0] = cur[i].x; w[4 * i + 1] = cur[i].y;
+            w[4 * i + 2] = cur[i].z; w[4 * i + 3] = cur[i].w;
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) to_planes<B>(&w[g * B], &pl[g * B]);
+        Sel sel[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) sel[d] = selectors(pl[d]);
+
+        const uint32_t *tt = tab_rg + (size_t)t * a.tab_stride_t;
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            if (j < (int)mrows) {
+                const uint32_t *tj = tt + (size_t)j * a.tab_stride_j;
+#pragma unroll
+                for (int p = 0; p < B; ++p) {
+#pragma unroll
+                    for (int q = 0; q < B; ++q) {
+                        const uint32_t *e = tj + (p * B + q) * TAB_DW;
+#pragma unroll
+                        for (int g = 0; g < G; ++g) acc[j][g * B + p] = cmul_acc(acc[j][g * B + p], e, sel[g * B + q]);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < P; ++i) cur[i] = nxt[i];
+    }
+
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        if (j < (int)mrows) {
+            uint32_t w[D];
+#pragma unroll
+            for (int g = 0; g < G; ++g) from_planes<B>(&acc[j][g * B], &w[g * B]);
+#pragma unroll
+            for (int i = 0; i < P; ++i)
+                if (ok[i])
+                    *(uint4 *)(a.dst[j0 + j] + off[i]) =
+                        make_uint4(w[4 * i + 0], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
+        }
+    }
+}
+
+// Generic kernel for heads, tails and unaligned buffers: one thread per group
+// of four elements, byte loads and stores, rows one at a time.
+template <int B, bool OW>
+__global__ __launch_bounds__(BLOCK) void k_matrix_generic(const MatrixArgs a) {
+    const uint64_t gid = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const uint64_t rel = gid * 4u * B;
+    if (rel >= a.ncols) return;
+    const uint32_t nel = (uint32_t)min<uint64_t>(4u, (a.ncols - rel) / B);
+    const uint32_t nbytes = nel * B;
+    const uint64_t base = a.col0 + rel;
+
+    for (uint32_t j = 0; j < a.m; ++j) {
+        uint32_t acc[B], w[B];
+        for (int d = 0; d < B; ++d) w[d] = 0;
+        if (!OW) {
+            const uint8_t *p = a.dst[j] + base;
+            for (uint32_t i = 0; i < nbytes; ++i) w[i >> 2] |= (uint32_t)p[i] << (8 * (i & 3));
+        }
+        to_planes<B>(w, acc);
+        for (uint32_t t = 0; t < a.k; ++t) {
+            uint32_t x[B], pl[B];
+            for (int d = 0; d < B; ++d) x[d] = 0;
+            const uint8_t *s = a.src[t] + base;
+            for (uint32_t i = 0; i < nbytes; ++i) x[i >> 2] |= (uint32_t)s[i] << (8 * (i & 3));
+            to_planes<B>(x, pl);
+            Sel sel[B];
+#pragma unroll
+            for (int q = 0; q < B; ++q) sel[q] = selectors(pl[q]);
+            const uint32_t *tj = a.tab + (size_t)t * a.tab_stride_t + (size_t)j * a.tab_stride_j;
+#pragma unroll
+            for (int p = 0; p < B; ++p)
+#pragma unroll
+                for (int q = 0; q < B; ++q) acc[p] = cmul_acc(acc[p], tj + (p * B + q) * TAB_DW, sel[q]);
+        }
+        from_planes<B>(acc, w);
+        uint8_t *o = a.dst[j] + base;
+        for (uint32_t i = 0; i < nbytes; ++i) o[i] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Kernel selection.
+// ---------------------------------------------------------------------------
+
+using MatrixFn = void (*)(MatrixArgs);
+
+// Rows per block for each element width, bounded by the accumulator budget.
+constexpr int max_rows(int nb) { return nb == 1 ? 16 : nb == 2 ? 8 : nb == 4 ? 4 : 2; }
+constexpr int dwords_for(int nb, int m) { return (nb == 1 && m > 8) ? 4 : 8; }
+
+template <int B, int M>
+MatrixFn pick_m(bool ow) {
+    constexpr int D = dwords_for(B, M);
+    return ow ? (MatrixFn)k_matrix<B, M, D, true> : (MatrixFn)k_matrix<B, M, D, false>;
+}
+
+template <int B, int... Ms>
+MatrixFn pick_from(int m, bool ow, std::integer_sequence<int, Ms...>) {
+    MatrixFn fn = nullptr;
+    ((m == Ms + 1 ? (fn = pick_m<B, Ms + 1>(ow), 0) : 0), ...);
+    return fn;
+}
+
+MatrixFn pick_vector(int nb, int m, bool ow) {
+    switch (nb) {
+    case 1: return pick_from<1>(m, ow, std::make_integer_sequence<int, 16>{});
+    case 2: return pick_from<2>(m, ow, std::make_integer_sequence<int, 8>{});
+    case 4: return pick_from<4>(m, ow, std::make_integer_sequence<int, 4>{});
+    default: return pick_from<8>(m, ow, std::make_integer_sequence<int, 2>{});
+    }
+}
+
+MatrixFn pick_generic(int nb, bool ow) {
+    switch (nb) {
+    case 1: return ow ? (MatrixFn)k_matrix_generic<1, true> : (MatrixFn)k_matrix_generic<1, false>;
+    case 2: return ow ? (MatrixFn)k_matrix_generic<2, true> : (MatrixFn)k_matrix_generic<2, false>;
+    case 4: return ow ? (MatrixFn)k_matrix_generic<4, true> : (MatrixFn)k_matrix_generic<4, false>;
+    default: return ow ? (MatrixFn)k_matrix_generic<8, true> : (MatrixFn)k_matrix_generic<8, false>;
+    }
+}
+
+hipError_t launch_segment(const MatrixJob &job, MatrixArgs a, size_t k0, size_t kn, size_t j0,
+                          size_t mn, uint64_t col0, uint64_t ncols, bool vector, bool ow) {
+    if (ncols == 0) return hipSuccess;
+    for (size_t t = 0; t < kn; ++t) a.src[t] = job.srcs[k0 + t];
+    for (size_t j = 0; j < mn; ++j) a.dst[j] = job.rows[j0 + j];
+    a.tab = job.tab + k0 * job.tab_stride_t + j0 * job.tab_stride_j;
+    a.tab_stride_t = (uint32_t)job.tab_stride_t;
+    a.tab_stride_j = (uint32_t)job.tab_stride_j;
+    a.k = (uint32_t)kn;
+    a.m = (uint32_t)mn;
+    a.col0 = col0;
+    a.ncols = ncols;
+    const int nb = job.nb;
+    if (vector) {
+        const int mt = std::min<int>((int)mn, max_rows(nb));
+        const int d = dwords_for(nb, mt);
+        const uint64_t block_bytes = (uint64_t)BLOCK / 64 * 256u * d;
+        a.ngroups = (uint32_t)((mn + mt - 1) / mt);
+        a.ncoltiles = (uint32_t)((ncols + block_bytes - 1) / block_bytes);
+        const uint64_t blocks = a.ngroups == 1
+                                    ? a.ncoltiles
+                                    : (uint64_t)((a.ncoltiles + 7) / 8) * 8u * a.ngroups;
+        hipLaunchKernelGGL(pick_vector(nb, mt, ow), dim3((uint32_t)blocks), dim3(BLOCK), 0,
+                           job.stream, a);
+    } else {
+        a.ngroups = 1;
+        const uint64_t groups = (ncols / nb + 3) / 4;
+        a.ncoltiles = (uint32_t)groups;
+        hipLaunchKernelGGL(pick_generic(nb, ow), dim3((uint32_t)((groups + BLOCK - 1) / BLOCK)),
+                           dim3(BLOCK), 0, job.stream, a);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_matrix(const MatrixJob &job) {
+    if (job.m == 0 || job.k == 0 || job.row_len == 0) return hipSuccess;
+    const int nb = job.nb;
+    // Column split: a common 16-byte misalignment is peeled as a head so the
+    // bulk runs on aligned 16-byte pieces; anything else runs generically.
+    const uintptr_t mis = (uintptr_t)job.srcs[0] & 15u;
+    bool same = true;
+    for (size_t t = 0; t < job.k; ++t) same &= (((uintptr_t)job.srcs[t] & 15u) == mis);
+    for (size_t j = 0; j < job.m; ++j) same &= (((uintptr_t)job.rows[j] & 15u) == mis);
+    uint64_t head = (16u - mis) & 15u;
+    uint64_t body = 0;
+    if (same && head % nb == 0 && head <= job.row_len) body = ((job.row_len - head) / 16u) * 16u;
+    if (body == 0) head = job.row_len;  // everything generic
+    const uint64_t tail0 = head + body, tail = job.row_len - tail0;
+
+    MatrixArgs a{};
+    for (size_t k0 = 0; k0 < job.k; k0 += KMAX) {
+        const size_t kn = std::min<size_t>(KMAX, job.k - k0);
+        const bool ow = job.overwrite && k0 == 0;
+        for (size_t j0 = 0; j0 < job.m; j0 += MMAX) {
+            const size_t mn = std::min<size_t>(MMAX, job.m - j0);
+            hipError_t e;
+            if ((e = launch_segment(job, a, k0, kn, j0, mn, 0, head, false, ow)) != hipSuccess) return e;
+            if ((e = launch_segment(job, a, k0, kn, j0, mn, head, body, true, ow)) != hipSuccess) return e;
+            if ((e = launch_segment(job, a, k0, kn, j0, mn, tail0, tail, false, ow)) != hipSuccess) return e;
+        }
+    }
+    return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise products.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+__constant__ ByteLogTables c_log_tables[3] = {make_log_tables(GF8B), make_log_tables(GF8D),
+                                              make_log_tables(FP8)};
+
+struct DevLog {
+    const uint8_t *ex;
+    const uint8_t *lg;
+    __device__ __forceinline__ uint32_t mul(uint32_t a, uint32_t b) const {
+        return (a && b) ? ex[lg[a] + lg[b]] : 0u;
+    }
+};
+
+// Karatsuba towers over Gf8B (gf16.rs:143-153, gf32.rs:121-129, gf64.rs:120-128).
+template <unsigned BITS>
+__device__ __forceinline__ uint64_t tower_mul(const DevLog &t, uint64_t x, uint64_t y) {
+    if constexpr (BITS == 8) {
+        return t.mul((uint32_t)x, (uint32_t)y);
+    } else {
+        constexpr unsigned H = BITS / 2;
+        constexpr uint64_t MASK = (1ull << H) - 1;
+        constexpr uint64_t DELTA = BITS == 16 ? 0x20u : BITS == 32 ? 0x2000u : 0x20000000ull;
+        const uint64_t x0 = x & MASK, x1 = x >> H, y0 = y & MASK, y1 = y >> H;
+        const uint64_t p00 = tower_mul<H>(t, x0, y0), p11 = tower_mul<H>(t, x1, y1);
+        const uint64_t lo = p00 ^ tower_mul<H>(t, DELTA, p11);
+        const uint64_t hi = tower_mul<H>(t, x0 ^ x1, y0 ^ y1) ^ p00;
+        return lo | (hi << H);
+    }
+}
+
+// Canonical Fan-Paar towers over fp8 (fan_paar.rs:124-145).
+template <unsigned BITS>
+__device__ __forceinline__ uint64_t fp_mul(const DevLog &t, uint64_t x, uint64_t y) {
+    if constexpr (BITS == 8) {
+        return t.mul((uint32_t)x, (uint32_t)y);
+    } else {
+        constexpr unsigned H = BITS / 2;
+        constexpr uint64_t MASK = (1ull << H) - 1;
+        const uint64_t x0 = x & MASK, x1 = x >> H, y0 = y & MASK, y1 = y >> H;
+        const uint64_t lo = fp_mul<H>(t, x0, y0), hi = fp_mul<H>(t, x1, y1);
+        const uint64_t mid = fp_mul<H>(t, x0 ^ x1, y0 ^ y1) ^ lo ^ hi;
+        return (lo ^ hi) | ((mid ^ fp_alpha(hi, H)) << H);
+    }
+}
+
+template <int F>
+__device__ __forceinline__ uint64_t elem_mul(const DevLog &t, uint64_t x, uint64_t y) {
+    constexpr unsigned BITS = 8u * field_bytes(F);
+    if constexpr (F == GF8B || F == GF8D || F == FP8) return t.mul((uint32_t)x, (uint32_t)y);
+    else if constexpr (is_fan_paar(F)) return fp_mul<BITS>(t, x, y);
+    else return tower_mul<BITS>(t, x, y);
+}
+
+template <int F>
+__device__ __forceinline__ void load_log_tables(uint8_t *s_ex, uint8_t *s_lg) {
+    constexpr int BF = byte_field(F);
+    constexpr int idx = BF == GF8B ? 0 : BF == GF8D ? 1 : 2;
+    const ByteLogTables &src = c_log_tables[idx];
+    for (int i = threadIdx.x; i < 512; i += BLOCK) s_ex[i] = src.exp[i];
+    for (int i = threadIdx.x; i < 256; i += BLOCK) s_lg[i] = src.log[i];
+    __syncthreads();
+}
+
+// 16 aligned bytes per thread per step; grid-stride over [0, n16) pieces.
+template <int F>
+__global__ __launch_bounds__(BLOCK) void k_elementwise(uint8_t *dst, const uint8_t *a,
+                                                        const uint8_t *b, uint64_t n16) {
+    __shared__ uint8_t s_ex[512], s_lg[256];
+    load_log_tables<F>(s_ex, s_lg);
+    const DevLog t{s_ex, s_lg};
+    constexpr int NB = field_bytes(F);
+    for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n16;
+         i += (uint64_t)gridDim.x * BLOCK) {
+        const uint4 va = ((const uint4 *)a)[i], vb = ((const uint4 *)b)[i];
+        uint32_t xa[4] = {va.x, va.y, va.z, va.w}, xb[4] = {vb.x, vb.y, vb.z, vb.w}, r[4];
+        if constexpr (NB == 1) {
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                uint32_t o = 0;
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    o |= t.mul((xa[d] >> (8 * k)) & 0xFF, (xb[d] >> (8 * k)) & 0xFF) << (8 * k);
+                r[d] = o;
+            }
+        } else if constexpr (NB == 2) {
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                const uint32_t lo = (uint32_t)elem_mul<F>(t, xa[d] & 0xFFFF, xb[d] & 0xFFFF);
+                const uint32_t hi = (uint32_t)elem_mul<F>(t, xa[d] >> 16, xb[d] >> 16);
+                r[d] = lo | (hi << 16);
+            }
+        } else if constexpr (NB == 4) {
+#pragma unroll
+            for (int d = 0; d < 4; ++d) r[d] = (uint32_t)elem_mul<F>(t, xa[d], xb[d]);
+        } else {
+#pragma unroll
+            for (int d = 0; d < 4; d += 2) {
+                const uint64_t p = elem_mul<F>(t, xa[d] | ((uint64_t)xa[d + 1] << 32),
+                                               xb[d] | ((uint64_t)xb[d + 1] << 32));
+                r[d] = (uint32_t)p;
+                r[d + 1] = (uint32_t)(p >> 32);
+            }
+        }
+        ((uint4 *)dst)[i] = make_uint4(r[0], r[1], r[2], r[3]);
+    }
+}
+
+// One element per thread with byte loads: heads, tails, unaligned buffers.
+template <int F>
+__global__ __launch_bounds__(BLOCK) void k_elementwise_generic(uint8_t *dst, const uint8_t *a,
+                                                                const uint8_t *b, uint64_t nelems) {
+    __shared__ uint8_t s_ex[512], s_lg[256];
+    load_log_tables<F>(s_ex, s_lg);
+    const DevLog t{s_ex, s_lg};
+    constexpr int NB = field_bytes(F);
+    for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < nelems;
+         i += (uint64_t)gridDim.x * BLOCK) {
+        uint64_t x = 0, y = 0;
+        for (int k = 0; k < NB; ++k) {
+            x |= (uint64_t)a[i * NB + k] << (8 * k);
+            y |= (uint64_t)b[i * NB + k] << (8 * k);
+        }
+        const uint64_t p = elem_mul<F>(t, x, y);
+        for (int k = 0; k < NB; ++k) dst[i * NB + k] = (uint8_t)(p >> (8 * k));
+    }
+}
+
+using EwFn = void (*)(uint8_t *, const uint8_t *, const uint8_t *, uint64_t);
+
+template <int... Fs>
+void pick_ew(int field, bool vector, EwFn &fn, std::integer_sequence<int, Fs...>) {
+    ((field == Fs ? (fn = vector ? (EwFn)k_elementwise<Fs> : (EwFn)k_elementwise_generic<Fs>, 0) : 0), ...);
+}
+
+unsigned grid_for(uint64_t items) {
+    const uint64_t blocks = (items + BLOCK - 1) / BLOCK;
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(blocks, 256u * 16u));
+}
+
+__global__ __launch_bounds__(BLOCK) void k_fill_zero(uint8_t *dst, uint64_t len) {
+    // Pieces of 16 bytes aligned to the buffer's 16-byte grid.
+    const uintptr_t base = (uintptr_t)dst & ~(uintptr_t)15;
+    const uintptr_t end = (uintptr_t)dst + len;
+    const uint64_t npieces = (end - base + 15) / 16;
+    for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < npieces;
+         i += (uint64_t)gridDim.x * BLOCK) {
+        const uintptr_t lo = base + i * 16, hi = lo + 16;
+        if (lo >= (uintptr_t)dst && hi <= end) {
+            *(uint4 *)lo = make_uint4(0, 0, 0, 0);
+        } else {
+            for (uintptr_t p = std::max(lo, (uintptr_t)dst); p < std::min(hi, end); ++p) *(uint8_t *)p = 0;
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t launch_elementwise(int field, uint8_t *dst, const uint8_t *a, const uint8_t *b,
+                              size_t len, hipStream_t stream) {
+    if (len == 0) return hipSuccess;
+    const int nb = field_bytes(field);
+    const uintptr_t mis = (uintptr_t)dst & 15u;
+    uint64_t head = len, body = 0;
+    if (((uintptr_t)a & 15u) == mis && ((uintptr_t)b & 15u) == mis) {
+        const uint64_t h = (16u - mis) & 15u;
+        if (h % nb == 0 && h <= len) {
+            head = h;
+            body = ((len - h) / 16u) * 16u;
+        }
+    }
+    const uint64_t tail0 = head + body, tail = len - tail0;
+    EwFn vec = nullptr, gen = nullptr;
+    pick_ew(field, true, vec, std::make_integer_sequence<int, FIELD_COUNT>{});
+    pick_ew(field, false, gen, std::make_integer_sequence<int, FIELD_COUNT>{});
+    if (head) {
+        hipLaunchKernelGGL(gen, dim3(grid_for(head / nb)), dim3(BLOCK), 0, stream, dst, a, b, head / nb);
+    }
+    if (body) {
+        hipLaunchKernelGGL(vec, dim3(grid_for(body / 16)), dim3(BLOCK), 0, stream, dst + head,
+                           a + head, b + head, body / 16);
+    }
+    if (tail) {
+        hipLaunchKernelGGL(gen, dim3(grid_for(tail / nb)), dim3(BLOCK), 0, stream, dst + tail0,
+                           a + tail0, b + tail0, tail / nb);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_zero(uint8_t *dst, size_t len, hipStream_t stream) {
+    if (len == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(len / 16 + 2)), dim3(BLOCK), 0, stream, dst,
+                       (uint64_t)len);
+    return hipGetLastError();
+}
+
+}  // namespace fgf

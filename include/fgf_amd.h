@@ -1,0 +1,207 @@
+/*
+ * fgf_amd: MI355X (gfx950) implementation of fgf's packed vector kernels,
+ * behind a C ABI.
+ *
+ * One entry point per function of the reference's checked `ops` façade
+ * (src/ops.rs). The reference is a generic Rust API over a field type F;
+ * here the field is the `fgf_field` argument and every `&[u8]` / `&mut [u8]`
+ * becomes a (pointer, length) pair so that the same geometry checks run with
+ * the same messages.
+ *
+ * Conventions
+ *  - Byte buffers (`rows`, `dst`, `src`, `srcs[i]`, `a`, `b`) are DEVICE
+ *    pointers (hipMalloc'd or otherwise device-accessible), little-endian
+ *    packed elements, any alignment. Lengths are in bytes.
+ *  - Coefficients are HOST memory: `ncoeffs` packed little-endian elements of
+ *    fgf_field_bytes(field) bytes each (what ops::pack writes). A term's
+ *    coefficient slice `(&[F::Elem], &[u8])` becomes term_coeffs[t] with
+ *    term_ncoeffs[t] elements.
+ *  - Work is enqueued on `stream` (a hipStream_t; NULL = the null stream)
+ *    and is asynchronous with respect to the host, like any HIP launch.
+ *  - Return value: FGF_OK, or a negative code. Where the reference panics
+ *    the call returns FGF_EINVAL, touches nothing, and fgf_last_error()
+ *    holds the reference's panic message (thread-local).
+ *  - Destination and source buffers must not overlap (the reference's
+ *    borrow rules); sources may alias each other. `fgf_mul_assign` is in
+ *    place by definition.
+ *  - The `_with` forms allocate nothing per call. Plans hold their tables in
+ *    device memory on the device that was current when they were built.
+ */
+#ifndef FGF_AMD_H
+#define FGF_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* The nine fields (src/lib.rs:137-148 re-exports; README.md:45-57). */
+typedef enum fgf_field {
+    FGF_GF8B = 0,  /* Gf8B: GF(2^8)/0x11B            src/field/gf8b.rs:97 */
+    FGF_GF8D = 1,  /* Gf8D: GF(2^8)/0x11D            src/field/gf8d.rs:40 */
+    FGF_GF16 = 2,  /* Gf16 tower over Gf8B           src/field/gf16.rs    */
+    FGF_GF32 = 3,  /* Gf32 tower over Gf16           src/field/gf32.rs    */
+    FGF_GF64 = 4,  /* Gf64 tower over Gf32           src/field/gf64.rs    */
+    FGF_FP8 = 5,   /* FanPaar8                       src/field/fan_paar.rs:536 */
+    FGF_FP16 = 6,  /* FanPaar16                      src/field/fan_paar.rs:548 */
+    FGF_FP32 = 7,  /* FanPaar32                      src/field/fan_paar.rs:563 */
+    FGF_FP64 = 8   /* FanPaar64                      src/field/fan_paar.rs:578 */
+} fgf_field;
+
+#define FGF_OK 0
+#define FGF_EINVAL (-1)  /* misuse: the reference would panic */
+#define FGF_EDEVICE (-2) /* HIP runtime failure */
+#define FGF_ENOMEM (-3)  /* device allocation failure */
+
+/* Thread-local text of the last error (the reference's panic message). */
+const char *fgf_last_error(void);
+
+/* Backend reporting: backend() / backend_for::<F>() / has_vector_elementwise
+ * (src/kernel/mod.rs:156-191). Always "hip-gfx950". */
+const char *fgf_backend(void);
+int fgf_has_vector_elementwise(fgf_field field);
+
+/* Field::BYTES and Field::NAME (src/field/mod.rs:111-144). 0 / "?" if invalid. */
+size_t fgf_field_bytes(fgf_field field);
+const char *fgf_field_name(fgf_field field);
+
+/* Scalar host arithmetic, Elem::mul / Elem::inv (src/field/mod.rs:42-108),
+ * for building coding matrices. Values are the raw little-endian integers. */
+int fgf_elem_mul(fgf_field field, uint64_t a, uint64_t b, uint64_t *out);
+int fgf_elem_inv(fgf_field field, uint64_t a, uint64_t *out);
+
+/* ------------------------------------------------------------------------
+ * Prepared coefficients: Coeff<F> (src/ops.rs:53-87) and Plan<F>
+ * (src/ops.rs:165-298). A Coeff is a plan of one coefficient; a CoeffRef
+ * (src/ops.rs:127-156) is (plan, index). Plans upload their tables to the
+ * current device once, at construction.
+ * ---------------------------------------------------------------------- */
+typedef struct fgf_plan fgf_plan;
+
+/* Plan::new (src/ops.rs:176-190): a 1 x ncoeffs vector. */
+int fgf_plan_new(fgf_field field, const void *coeffs, size_t ncoeffs, fgf_plan **out);
+/* Plan::matrix (src/ops.rs:198-212): row-major rows x cols, (t, j) at t*cols + j. */
+int fgf_plan_matrix(fgf_field field, size_t rows, size_t cols, const void *coeffs,
+                    size_t ncoeffs, fgf_plan **out);
+/* Coeff::new (src/ops.rs:61-65). */
+int fgf_coeff_new(fgf_field field, uint64_t coeff, fgf_plan **out);
+void fgf_plan_free(fgf_plan *plan);
+/* Plan::len, Plan::dimensions, Plan::get(i).value(), Plan::get_at(r, c)
+ * (src/ops.rs:216-251); fgf_plan_get returns FGF_EINVAL when out of range
+ * (the reference returns None). */
+size_t fgf_plan_len(const fgf_plan *plan);
+void fgf_plan_dimensions(const fgf_plan *plan, size_t *rows, size_t *cols);
+fgf_field fgf_plan_field(const fgf_plan *plan);
+int fgf_plan_get(const fgf_plan *plan, size_t index, uint64_t *value);
+int fgf_plan_get_at(const fgf_plan *plan, size_t row, size_t col, uint64_t *value);
+
+/* ------------------------------------------------------------------------
+ * Single-row operations (src/ops.rs:323-450).
+ * ---------------------------------------------------------------------- */
+/* add_assign / sub_assign (src/ops.rs:323-335): dst ^= src. */
+int fgf_add_assign(fgf_field field, uint8_t *dst, size_t dst_len, const uint8_t *src,
+                   size_t src_len, void *stream);
+/* mul_add (src/ops.rs:342-352): dst ^= coeff * src. */
+int fgf_mul_add(fgf_field field, uint8_t *dst, size_t dst_len, uint64_t coeff,
+                const uint8_t *src, size_t src_len, void *stream);
+/* mul_add_with (src/ops.rs:359-374), coefficient = plan entry `index`. */
+int fgf_mul_add_with(fgf_field field, uint8_t *dst, size_t dst_len, const fgf_plan *plan,
+                     size_t index, const uint8_t *src, size_t src_len, void *stream);
+/* mul_into (src/ops.rs:381-392): dst = coeff * src. */
+int fgf_mul_into(fgf_field field, uint8_t *dst, size_t dst_len, uint64_t coeff,
+                 const uint8_t *src, size_t src_len, void *stream);
+/* mul_into_with (src/ops.rs:399-415). */
+int fgf_mul_into_with(fgf_field field, uint8_t *dst, size_t dst_len, const fgf_plan *plan,
+                      size_t index, const uint8_t *src, size_t src_len, void *stream);
+/* mul_assign (src/ops.rs:422-432): dst *= coeff, in place. */
+int fgf_mul_assign(fgf_field field, uint8_t *dst, size_t dst_len, uint64_t coeff, void *stream);
+/* mul_assign_with (src/ops.rs:439-450). */
+int fgf_mul_assign_with(fgf_field field, uint8_t *dst, size_t dst_len, const fgf_plan *plan,
+                        size_t index, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Multi-row shapes (src/ops.rs:461-941) — the hot path.
+ * ---------------------------------------------------------------------- */
+/* mul_add_scatter (src/ops.rs:461-482): rows[j] ^= coeffs[j] * src. */
+int fgf_mul_add_scatter(fgf_field field, uint8_t *rows, size_t rows_len, size_t row_len,
+                        const void *coeffs, size_t ncoeffs, const uint8_t *src, size_t src_len,
+                        void *stream);
+/* mul_add_scatter_with (src/ops.rs:493-520). */
+int fgf_mul_add_scatter_with(fgf_field field, uint8_t *rows, size_t rows_len, size_t row_len,
+                             const fgf_plan *plan, const uint8_t *src, size_t src_len,
+                             void *stream);
+/* mul_add_gather (src/ops.rs:533-555): dst ^= sum coeffs[i] * srcs[i]. */
+int fgf_mul_add_gather(fgf_field field, uint8_t *dst, size_t dst_len, const void *coeffs,
+                       size_t ncoeffs, const uint8_t *const *srcs, const size_t *src_lens,
+                       size_t nsrcs, void *stream);
+/* mul_add_gather_with (src/ops.rs:563-585). */
+int fgf_mul_add_gather_with(fgf_field field, uint8_t *dst, size_t dst_len, const fgf_plan *plan,
+                            const uint8_t *const *srcs, const size_t *src_lens, size_t nsrcs,
+                            void *stream);
+/* dot_product (src/ops.rs:597-624): dst = sum coeffs[i] * srcs[i]. */
+int fgf_dot_product(fgf_field field, uint8_t *dst, size_t dst_len, const void *coeffs,
+                    size_t ncoeffs, const uint8_t *const *srcs, const size_t *src_lens,
+                    size_t nsrcs, void *stream);
+/* dot_product_with (src/ops.rs:632-660). */
+int fgf_dot_product_with(fgf_field field, uint8_t *dst, size_t dst_len, const fgf_plan *plan,
+                         const uint8_t *const *srcs, const size_t *src_lens, size_t nsrcs,
+                         void *stream);
+/* mul_add_matrix (src/ops.rs:675-708): rows[j] ^= sum_t term_coeffs[t][j] * term_srcs[t]. */
+int fgf_mul_add_matrix(fgf_field field, uint8_t *rows, size_t rows_len, size_t row_len,
+                       size_t nrows, const void *const *term_coeffs, const size_t *term_ncoeffs,
+                       const uint8_t *const *term_srcs, const size_t *term_src_lens,
+                       size_t nterms, void *stream);
+/* mul_add_matrix_with (src/ops.rs:720-760): plan dimensions (nsrcs, nrows). */
+int fgf_mul_add_matrix_with(fgf_field field, uint8_t *rows, size_t rows_len, size_t row_len,
+                            size_t nrows, const fgf_plan *plan, const uint8_t *const *srcs,
+                            const size_t *src_lens, size_t nsrcs, void *stream);
+/* dot_product_matrix (src/ops.rs:773-810): rows[j] = sum_t term_coeffs[t][j] * term_srcs[t]. */
+int fgf_dot_product_matrix(fgf_field field, uint8_t *rows, size_t rows_len, size_t row_len,
+                           size_t nrows, const void *const *term_coeffs,
+                           const size_t *term_ncoeffs, const uint8_t *const *term_srcs,
+                           const size_t *term_src_lens, size_t nterms, void *stream);
+/* dot_product_matrix_with (src/ops.rs:823-867) — the RS encode entry point. */
+int fgf_dot_product_matrix_with(fgf_field field, uint8_t *rows, size_t rows_len,
+                                size_t row_len, size_t nrows, const fgf_plan *plan,
+                                const uint8_t *const *srcs, const size_t *src_lens,
+                                size_t nsrcs, void *stream);
+/* mul_add_matrix_scattered (src/ops.rs:887-941): row j at dst + row_starts[j]. */
+int fgf_mul_add_matrix_scattered(fgf_field field, uint8_t *dst, size_t dst_len, size_t row_len,
+                                 const size_t *row_starts, size_t nrows,
+                                 const void *const *term_coeffs, const size_t *term_ncoeffs,
+                                 const uint8_t *const *term_srcs, const size_t *term_src_lens,
+                                 size_t nterms, void *stream);
+/* Prepared form of the scattered shape: the reference has none; this is
+ * mul_add_matrix_scattered with a (nsrcs, nrows) plan, so decode matrices can
+ * stay resident on the device. Same checks and messages as the one-shot form. */
+int fgf_mul_add_matrix_scattered_with(fgf_field field, uint8_t *dst, size_t dst_len,
+                                      size_t row_len, const size_t *row_starts, size_t nrows,
+                                      const fgf_plan *plan, const uint8_t *const *srcs,
+                                      const size_t *src_lens, size_t nsrcs, void *stream);
+/* mul_elementwise (src/ops.rs:953-957): dst[i] = a[i] * b[i]. */
+int fgf_mul_elementwise(fgf_field field, uint8_t *dst, size_t dst_len, const uint8_t *a,
+                        size_t a_len, const uint8_t *b, size_t b_len, void *stream);
+
+/* Diagnostic: the device table of one coefficient as the kernels read it,
+ * B*B entries of 5 dwords {T0 lo, T0 hi, T1 lo, T1 hi, T2}, entry (p, q) for
+ * output byte p from input byte q, with l*x = T0[x & 7] ^ T1[(x >> 3) & 7] ^
+ * T2[x >> 6] per byte. No counterpart in the reference (its prepared forms
+ * are private, src/kernel/tables.rs); it lets a CPU model check the table
+ * math. `out` holds fgf_field_bytes(field)^2 * 5 dwords. */
+int fgf_coeff_table(fgf_field field, uint64_t coeff, uint32_t *out);
+
+/* ------------------------------------------------------------------------
+ * Element packing (src/ops.rs:963-1017), host memory only.
+ * ---------------------------------------------------------------------- */
+/* pack: elems (raw integers) -> little-endian bytes. */
+int fgf_pack(fgf_field field, uint8_t *dst, size_t dst_len, const uint64_t *elems, size_t nelems);
+/* unpack: little-endian bytes -> raw integers. */
+int fgf_unpack(fgf_field field, uint64_t *dst, size_t nelems, const uint8_t *src, size_t src_len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FGF_AMD_H */

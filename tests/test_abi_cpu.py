@@ -1,0 +1,271 @@
+"""CPU-side checks of the product library: exports, host arithmetic, the
+coefficient-table math (through a numpy model of the device kernel's
+v_perm lookups and byte-plane transforms), packing, and every geometry check
+with the reference's panic text. No GPU is touched: validation runs before
+anything is enqueued, so misuse is testable on the host.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+import oracle_lib as ol
+from fgf_amd import ALL_FIELDS, BY_NAME, FanPaar16, Gf8B, Gf8D, Gf16, Panic, lib, ops
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+NAMES = {f.id: n for n, f in BY_NAME.items()}
+
+
+def test_library_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, "include", "fgf_amd.h")).read()
+    declared = set(re.findall(r"\b(fgf_[a-z0-9_]+)\s*\(", header))
+    from fgf_amd._lib import PROTOTYPES
+    assert declared == set(PROTOTYPES), declared ^ set(PROTOTYPES)
+    l = lib()
+    for name in sorted(declared):
+        assert hasattr(l, name), name
+
+
+def test_backend_reporting():
+    import fgf_amd
+    assert fgf_amd.backend() == "hip-gfx950"
+    assert all(fgf_amd.has_vector_elementwise(f) for f in ALL_FIELDS)
+
+
+def test_field_constants_match_reference_names():
+    for f in ALL_FIELDS:
+        assert lib().fgf_field_bytes(f.id) == f.BYTES
+        assert lib().fgf_field_name(f.id).decode() == f.NAME
+
+
+@pytest.mark.parametrize("f", ALL_FIELDS, ids=lambda f: NAMES[f.id])
+def test_host_arithmetic_matches_oracle(f):
+    name = NAMES[f.id]
+    if f.BYTES == 1:
+        table = ol.product_table(name, xtime=False)
+        for a in range(256):
+            for b in range(0, 256, 5):
+                assert f.mul(a, b) == table[a, b]
+    rng = np.random.default_rng(f.id)
+    for _ in range(200):
+        a, b = (int(x) for x in rng.integers(0, 1 << min(f.BITS, 63), 2, dtype=np.uint64))
+        if f.BITS == 64:
+            a |= int(rng.integers(0, 2)) << 63
+        assert f.mul(a, b) == ol.mul(name, a, b), (name, hex(a), hex(b))
+        if a:
+            assert f.inv(a) == ol.inv(name, a)
+
+
+# ---------------------------------------------------------------------------
+# Numpy model of the device arithmetic: v_perm_b32 byte selection and the
+# byte-plane transforms, applied to the library's own coefficient tables.
+# ---------------------------------------------------------------------------
+
+def vperm(hi, lo, sel):
+    """v_perm_b32 for selector bytes 0..7 (the only values the kernels use)."""
+    pool = np.concatenate([lo.view(np.uint8).reshape(-1, 4), hi.view(np.uint8).reshape(-1, 4)], 1)
+    s = sel.view(np.uint8).reshape(-1, 4).astype(np.int64)
+    assert s.max() < 8
+    out = np.take_along_axis(pool, s, axis=1)
+    return out.reshape(-1).view(np.uint32)
+
+
+def coeff_table(f, c):
+    n = f.BYTES * f.BYTES * 5
+    out = (ctypes.c_uint32 * n)()
+    assert lib().fgf_coeff_table(f.id, c, out) == 0
+    return np.array(out, dtype=np.uint32).reshape(f.BYTES, f.BYTES, 5)
+
+
+def model_mul(f, c, data: np.ndarray) -> np.ndarray:
+    """c * data (packed elements) the way the kernel computes it."""
+    B = f.BYTES
+    tab = coeff_table(f, c)
+    x = data.reshape(-1, B)                        # elements
+    planes = [x[:, q].copy() for q in range(B)]    # byte q of every element
+    out = []
+    for p in range(B):
+        acc = np.zeros(len(x) // 4 * 4 // 4, dtype=np.uint32) if False else None
+        acc = np.zeros(x.shape[0] // 4, dtype=np.uint32)
+        for q in range(B):
+            v = planes[q].view(np.uint32)          # four elements per dword
+            s0 = v & np.uint32(0x07070707)
+            s1 = (v >> np.uint32(3)) & np.uint32(0x07070707)
+            s2 = (v >> np.uint32(6)) & np.uint32(0x03030303)
+            e = tab[p, q]
+            n = len(v)
+            acc ^= vperm(np.full(n, e[1], np.uint32), np.full(n, e[0], np.uint32), s0)
+            acc ^= vperm(np.full(n, e[3], np.uint32), np.full(n, e[2], np.uint32), s1)
+            acc ^= vperm(np.full(n, e[4], np.uint32), np.full(n, e[4], np.uint32), s2)
+        out.append(acc.view(np.uint8))
+    return np.stack(out, 1).reshape(-1)
+
+
+@pytest.mark.parametrize("f", ALL_FIELDS, ids=lambda f: NAMES[f.id])
+def test_coefficient_tables_reproduce_field_products(f):
+    name = NAMES[f.id]
+    n = 64 * f.BYTES
+    data = ol.noise(n, 0x7AB + f.id)
+    if f.BYTES == 1:
+        data = np.arange(256, dtype=np.uint8)      # every byte value
+        coeffs = range(256)
+    else:
+        rng = np.random.default_rng(11)
+        coeffs = [0, 1, 2] + [int(v) for v in rng.integers(0, 1 << min(f.BITS, 62), 24, dtype=np.uint64)]
+    for c in coeffs:
+        want = np.zeros_like(data)
+        ol.mul_add(name, want, c, data)
+        assert np.array_equal(model_mul(f, c, data), want), (name, hex(c))
+
+
+def test_planar_transposes_are_inverse_permutations():
+    # The byte selectors of the in-register transforms (kernels.hip to_planes /
+    # from_planes): applying them to labelled bytes must land byte p of
+    # element e in plane p, lane e, and invert exactly.
+    def transpose4(a, b, c, d):
+        t0, t1 = vperm(b, a, U(0x05010400)), vperm(b, a, U(0x07030602))
+        t2, t3 = vperm(d, c, U(0x05010400)), vperm(d, c, U(0x07030602))
+        return (vperm(t2, t0, U(0x05040100)), vperm(t2, t0, U(0x07060302)),
+                vperm(t3, t1, U(0x05040100)), vperm(t3, t1, U(0x07060302)))
+
+    def U(v):
+        return np.array([v], dtype=np.uint32)
+
+    for B in (2, 4, 8):
+        raw = np.arange(4 * B, dtype=np.uint8)       # element e byte p = e*B + p
+        w = [raw[4 * i:4 * i + 4].view(np.uint32) for i in range(B)]
+        if B == 2:
+            pl = [vperm(w[1], w[0], U(0x06040200)), vperm(w[1], w[0], U(0x07050301))]
+            back = [vperm(pl[1], pl[0], U(0x05010400)), vperm(pl[1], pl[0], U(0x07030602))]
+        elif B == 4:
+            pl = list(transpose4(*w))
+            back = list(transpose4(*pl))
+        else:
+            lo = transpose4(w[0], w[2], w[4], w[6])
+            hi = transpose4(w[1], w[3], w[5], w[7])
+            pl = list(lo) + list(hi)
+            a = transpose4(*pl[:4])
+            b = transpose4(*pl[4:])
+            back = [a[0], b[0], a[1], b[1], a[2], b[2], a[3], b[3]]
+        for p in range(B):
+            assert pl[p].view(np.uint8).tolist() == [e * B + p for e in range(4)], (B, p)
+        assert np.concatenate([x.view(np.uint8) for x in back]).tolist() == raw.tolist()
+
+
+# ---------------------------------------------------------------------------
+# Packing (tests/ops.rs:813-832).
+# ---------------------------------------------------------------------------
+
+def test_pack_unpack_round_trip():
+    elems = [0, 1, 0x0108, 0xFFFF]
+    b = ops.pack_to_vec(Gf16, elems)
+    assert len(b) == 8 and bytes(b) == b"\x00\x00\x01\x00\x08\x01\xff\xff"
+    dec = [0] * 4
+    ops.unpack(Gf16, dec, bytes(b))
+    assert dec == elems
+    with pytest.raises(Panic, match="pack: dst is 7 bytes but 4 GF\\(2\\^16\\) elements need 8"):
+        ops.pack(Gf16, bytearray(7), elems)
+    with pytest.raises(Panic, match="unpack: src is 3 bytes"):
+        ops.unpack(Gf16, [0, 0], b"abc")
+
+
+# ---------------------------------------------------------------------------
+# Geometry validation with the reference's panic text (tests/ops.rs:500-523,
+# 950-1014, plus the remaining assertions of src/ops.rs). CPU tensors stand
+# in for device buffers: every call below fails before touching them.
+# ---------------------------------------------------------------------------
+
+def z(n):
+    return torch.zeros(n, dtype=torch.uint8)
+
+
+def test_panic_mul_add_length_mismatch():
+    with pytest.raises(Panic, match="dst is 8 bytes but src is 4 bytes"):
+        ops.mul_add(Gf8B, z(8), 2, z(4))
+
+
+def test_panic_gf16_odd_length():
+    with pytest.raises(Panic, match="whole number of"):
+        ops.mul_add(Gf16, z(7), 2, z(7))
+    with pytest.raises(Panic, match="mul_assign: buffer of 7 bytes is not a whole number of GF\\(2\\^16\\) elements"):
+        ops.mul_assign(Gf16, z(7), 2)
+
+
+def test_panic_scatter_rows_too_short():
+    with pytest.raises(Panic, match="rows is 30 bytes"):
+        ops.mul_add_scatter(Gf8B, z(30), 8, [1] * 4, z(8))
+    with pytest.raises(Panic, match="mul_add_scatter: rows is 30 bytes but 4 rows of 8 bytes need 32"):
+        ops.mul_add_scatter(Gf8B, z(30), 8, [1] * 4, z(8))
+
+
+def test_panic_matrix_coefficient_count():
+    with pytest.raises(Panic, match="coefficients for"):
+        ops.mul_add_matrix(Gf8B, z(32), 8, 4, [([1, 1], z(8))])
+    with pytest.raises(Panic, match="dot_product_matrix: term supplies 2 coefficients for 4 rows"):
+        ops.dot_product_matrix(Gf8B, z(32), 8, 4, [([1, 1], z(8))])
+
+
+def test_panic_matrix_source_length_and_rows():
+    with pytest.raises(Panic, match="mul_add_matrix: source is 7 bytes, expected 8"):
+        ops.mul_add_matrix(Gf8B, z(32), 8, 4, [([1] * 4, z(7))])
+    with pytest.raises(Panic, match="dot_product_matrix: rows is 31 bytes but 4 rows of 8 bytes need 32"):
+        ops.dot_product_matrix(Gf8B, z(31), 8, 4, [([1] * 4, z(8))])
+
+
+def test_panic_dot_product_counts():
+    with pytest.raises(Panic, match="dot_product: 2 coefficients for 1 sources"):
+        ops.dot_product(Gf8B, z(8), [2, 3], [z(8)])
+    with pytest.raises(Panic, match="dot_product: source 0 is 7 bytes, expected 8"):
+        ops.dot_product(Gf8B, z(8), [2], [z(7)])
+    with pytest.raises(Panic, match="mul_add_gather: source is 7 bytes, expected 8"):
+        ops.mul_add_gather(Gf8B, z(8), [2], [z(7)])
+    with pytest.raises(Panic, match="mul_add_gather: 1 coefficients for 2 sources"):
+        ops.mul_add_gather(Gf8B, z(8), [2], [z(8), z(8)])
+
+
+def test_panic_scattered_rows():
+    with pytest.raises(Panic, match="overlap"):
+        ops.mul_add_matrix_scattered(Gf8B, z(64), 16, [0, 8], [([1, 1], z(16))])
+    with pytest.raises(Panic, match="but dst is"):
+        ops.mul_add_matrix_scattered(Gf8B, z(32), 16, [0, 24], [([1, 1], z(16))])
+    with pytest.raises(Panic, match="coefficients for"):
+        ops.mul_add_matrix_scattered(Gf8B, z(64), 16, [0, 16, 32], [([1, 1], z(16))])
+    with pytest.raises(Panic, match="row 1 offset 3 is not a whole number of GF\\(2\\^16\\) elements"):
+        ops.mul_add_matrix_scattered(Gf16, z(64), 16, [20, 3], [([1, 1], z(16))])
+    with pytest.raises(Panic, match="rows at 0 and 8 overlap for 16-byte rows"):
+        ops.mul_add_matrix_scattered(Gf8B, z(64), 16, [0, 8], [([1, 1], z(16))])
+
+
+def test_panic_elementwise_lengths():
+    with pytest.raises(Panic, match="mul_elementwise: dst is 8 bytes but a is 6 bytes"):
+        ops.mul_elementwise(Gf8D, z(8), z(6), z(8))
+    with pytest.raises(Panic, match="mul_elementwise: dst is 8 bytes but b is 9 bytes"):
+        ops.mul_elementwise(Gf8D, z(8), z(8), z(9))
+    with pytest.raises(Panic, match="whole number of Fan-Paar GF\\(2\\^16\\) elements"):
+        ops.mul_elementwise(FanPaar16, z(3), z(3), z(3))
+
+
+def test_panic_scatter_width():
+    with pytest.raises(Panic, match="mul_add_scatter: row_len is 8 bytes but src is 7 bytes"):
+        ops.mul_add_scatter(Gf8D, z(64), 8, [1], z(7))
+
+
+def test_coefficients_must_be_field_elements():
+    with pytest.raises(Panic, match="not an element"):
+        ops.mul_add(Gf8D, z(8), 0x100, z(8))
+
+
+def test_zero_nrows_and_empty_inputs_are_host_side_no_ops():
+    # src/ops.rs:704-706, 802-804, 937-939, 551-553, 478-480: nothing is
+    # enqueued, so these return without a device (tests/ops.rs:1016-1025).
+    e = z(0)
+    ops.mul_add_matrix(Gf8B, e, 8, 0, [])
+    ops.dot_product_matrix(Gf8B, e, 8, 0, [])
+    ops.mul_add_gather(Gf8B, e, [], [])
+    ops.mul_add_scatter(Gf8B, e, 0, [], e)
+    ops.mul_add(Gf8B, z(16), 0, z(16))
+    ops.mul_assign(Gf16, e, 7)
+    ops.mul_add_matrix_scattered(Gf8B, z(32), 16, [], [])
