@@ -29,6 +29,7 @@ PROTOTYPES = {
     "fgf_field_name": (ctypes.c_char_p, [c_int]),
     "fgf_elem_mul": (c_int, [c_int, c_u64, c_u64, ctypes.POINTER(c_u64)]),
     "fgf_elem_inv": (c_int, [c_int, c_u64, ctypes.POINTER(c_u64)]),
+    "fgf_matrix_invert": (c_int, [c_int, c_size, c_vp, c_vp]),
     "fgf_plan_new": (c_int, [c_int, c_vp, c_size, ctypes.POINTER(c_plan)]),
     "fgf_plan_matrix": (c_int, [c_int, c_size, c_size, c_vp, c_size, ctypes.POINTER(c_plan)]),
     "fgf_coeff_new": (c_int, [c_int, c_u64, ctypes.POINTER(c_plan)]),
@@ -65,6 +66,13 @@ PROTOTYPES = {
                                                   c_plan, c_ptrs, c_sizes, c_size, c_vp]),
     "fgf_mul_elementwise": (c_int, [c_int, c_u8p, c_size, c_u8p, c_size, c_u8p, c_size, c_vp]),
     "fgf_coeff_table": (c_int, [c_int, c_u64, ctypes.POINTER(ctypes.c_uint32)]),
+    "fgf_batch_create": (c_int, [c_int, c_vp, c_size, ctypes.POINTER(c_vp)]),
+    "fgf_batch_configure": (c_int, [c_vp, c_int, c_int, c_int]),
+    "fgf_batch_run": (c_int, [c_vp, c_vp]),
+    "fgf_batch_run_graph": (c_int, [c_vp, c_vp]),
+    "fgf_batch_items": (c_size, [c_vp]),
+    "fgf_batch_bytes": (ctypes.c_uint64, [c_vp]),
+    "fgf_batch_free": (None, [c_vp]),
     "fgf_pack": (c_int, [c_int, c_vp, c_size, ctypes.POINTER(c_u64), c_size]),
     "fgf_unpack": (c_int, [c_int, ctypes.POINTER(c_u64), c_size, c_vp, c_size]),
 }

@@ -1,0 +1,78 @@
+"""Batched launches: many independent stripes in one persistent kernel.
+
+Each stripe is a (k, m) plan, k source buffers, m destination rows, a row
+length and a mode: overwrite (dot_product_matrix_with semantics) or
+accumulate (mul_add_matrix_with semantics). `Batch.run()` executes all of
+them in one launch from a device-resident descriptor queue;
+`Batch.run_graph()` replays a captured hipGraph of that launch.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Sequence
+
+from . import _lib
+from .field import Field, _check
+from .ops import Plan, _buf, _stream
+
+
+class _CStripe(ctypes.Structure):
+    _fields_ = [("plan", ctypes.c_void_p), ("srcs", ctypes.c_void_p), ("rows", ctypes.c_void_p),
+                ("row_len", ctypes.c_size_t), ("overwrite", ctypes.c_int)]
+
+
+@dataclass
+class Stripe:
+    plan: Plan
+    srcs: Sequence          # k device byte buffers
+    rows: Sequence          # m device byte buffers
+    row_len: int
+    overwrite: bool = True
+
+
+class Batch:
+    def __init__(self, field: Field, stripes: Sequence[Stripe]):
+        self.field = field
+        self.stripes = list(stripes)          # keeps plans and buffers alive
+        n = len(self.stripes)
+        arr = (_CStripe * max(n, 1))()
+        self._keep = []
+        for i, s in enumerate(self.stripes):
+            sp = (ctypes.c_void_p * max(len(s.srcs), 1))(*[_buf(x)[0] for x in s.srcs])
+            rp = (ctypes.c_void_p * max(len(s.rows), 1))(*[_buf(x)[0] for x in s.rows])
+            self._keep += [sp, rp]
+            arr[i] = _CStripe(s.plan._h.value, ctypes.addressof(sp), ctypes.addressof(rp),
+                              s.row_len, 1 if s.overwrite else 0)
+        h = ctypes.c_void_p()
+        _check(_lib.lib().fgf_batch_create(field.id, ctypes.addressof(arr), n, ctypes.byref(h)))
+        self._h = h
+
+    def configure(self, dwords: int = 8, waves_per_cu: int = 16, order: int = 1) -> None:
+        _check(_lib.lib().fgf_batch_configure(self._h, dwords, waves_per_cu, order))
+
+    def _s(self, stream):
+        if stream is not None:
+            return int(stream)
+        bufs = [b for s in self.stripes for b in s.rows[:1]]
+        return _stream(None, *bufs)
+
+    def run(self, stream=None) -> None:
+        _check(_lib.lib().fgf_batch_run(self._h, self._s(stream)))
+
+    def run_graph(self, stream=None) -> None:
+        _check(_lib.lib().fgf_batch_run_graph(self._h, self._s(stream)))
+
+    @property
+    def items(self) -> int:
+        return _lib.lib().fgf_batch_items(self._h)
+
+    @property
+    def bytes(self) -> int:
+        return _lib.lib().fgf_batch_bytes(self._h)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib._lib is not None:
+            _lib.lib().fgf_batch_free(h)
+            self._h = None

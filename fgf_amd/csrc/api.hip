@@ -15,15 +15,7 @@
 #include "../../include/fgf_amd.h"
 #include "field.hpp"
 #include "kernels.hpp"
-
-struct fgf_plan {
-    int field = 0;
-    int nb = 1;
-    size_t rows = 0, cols = 0;
-    std::vector<uint64_t> values;
-    uint32_t *d_tab = nullptr;  // entry i at i * nb * nb * TAB_DW dwords
-    int device = 0;
-};
+#include "plan.hpp"
 
 namespace {
 
@@ -41,6 +33,12 @@ int fail(const char *fmt, ...) {
     g_err = buf;
     return FGF_EINVAL;
 }
+
+}  // namespace
+
+void fgf::set_last_error(const char *msg) { g_err = msg; }
+
+namespace {
 
 int device_fail(const char *what, hipError_t e) {
     g_err = std::string(what) + ": " + hipGetErrorString(e);
@@ -351,6 +349,44 @@ int fgf_elem_inv(fgf_field f, uint64_t a, uint64_t *out) {
     int rc = 0;
     if (!coeff_in_field(f, a, "elem_inv", &rc)) return rc;
     *out = HostField::inv(f, a);
+    return FGF_OK;
+}
+
+int fgf_matrix_invert(fgf_field f, size_t n, const void *in, void *out) {
+    FIELD_GUARD(f);
+    const int nb = field_bytes(f);
+    size_t nn;
+    if (!checked_mul(n, n, &nn)) return fail("matrix_invert: dimensions overflow");
+    std::vector<uint64_t> a, inv(nn, 0);
+    int rc = 0;
+    if (!decode_coeffs(f, in, nn, a, &rc)) return rc;
+    for (size_t i = 0; i < n; ++i) inv[i * n + i] = 1;
+    for (size_t col = 0; col < n; ++col) {
+        size_t piv = col;
+        while (piv < n && a[piv * n + col] == 0) ++piv;
+        if (piv == n) return fail("matrix_invert: matrix is singular (column %zu)", col);
+        if (piv != col)
+            for (size_t c = 0; c < n; ++c) {
+                std::swap(a[col * n + c], a[piv * n + c]);
+                std::swap(inv[col * n + c], inv[piv * n + c]);
+            }
+        const uint64_t s = HostField::inv(f, a[col * n + col]);
+        for (size_t c = 0; c < n; ++c) {
+            a[col * n + c] = HostField::mul(f, a[col * n + c], s);
+            inv[col * n + c] = HostField::mul(f, inv[col * n + c], s);
+        }
+        for (size_t r = 0; r < n; ++r) {
+            const uint64_t fac = a[r * n + col];
+            if (r == col || fac == 0) continue;
+            for (size_t c = 0; c < n; ++c) {
+                a[r * n + c] ^= HostField::mul(f, fac, a[col * n + c]);
+                inv[r * n + c] ^= HostField::mul(f, fac, inv[col * n + c]);
+            }
+        }
+    }
+    uint8_t *o = static_cast<uint8_t *>(out);
+    for (size_t i = 0; i < nn; ++i)
+        for (int k = 0; k < nb; ++k) o[i * nb + k] = (uint8_t)(inv[i] >> (8 * k));
     return FGF_OK;
 }
 

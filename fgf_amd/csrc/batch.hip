@@ -1,0 +1,349 @@
+// Batched matrix operations: thousands of independent stripes (each its own
+// k, m, row length, coefficient plan and overwrite/accumulate mode) in ONE
+// persistent kernel launch.
+//
+// The launch reads a device-resident descriptor table: one StripeDesc per
+// stripe and a work list of (stripe, column chunk, row group) items. Every
+// wave pulls items from a device counter with one atomic per item, so uneven
+// stripes balance themselves across the 256 CUs. The counter resets itself:
+// the last wave to retire zeroes it, so a captured hipGraph of the launch
+// replays with no memset node. The per-item body is the same register-tiled
+// perm kernel as the per-call path (matrix_core.hpp).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/fgf_amd.h"
+#include "batch.hpp"
+#include "field.hpp"
+#include "matrix_core.hpp"
+
+namespace fgf {
+namespace {
+
+constexpr int BATCH_BLOCK = 256;
+
+template <int B, int M, int D>
+__global__ __launch_bounds__(BATCH_BLOCK) void k_batch(const BatchArgs a) {
+    const uint32_t lane = threadIdx.x & 63u;
+    constexpr uint64_t CHUNK = 256u * D;
+    for (;;) {
+        uint32_t it = 0;
+        if (lane == 0) it = atomicAdd(&a.counter[0], 1u);
+        it = __builtin_amdgcn_readfirstlane(it);
+        if (it >= a.nitems) break;
+        const WorkItem w = a.items[it];
+        const StripeDesc s = a.stripes[w.stripe];
+        const uint32_t j0 = w.rowgroup * M;
+        const uint32_t mrows = min((uint32_t)M, s.m - j0);
+        core::matrix_tile<B, M, D>(s.srcs, s.rows + j0, s.tab + (size_t)j0 * s.tab_stride_j,
+                                   s.tab_stride_t, s.tab_stride_j, s.k, mrows, s.overwrite != 0, 0,
+                                   s.row_len, (uint64_t)w.chunk * CHUNK, lane);
+    }
+    // Retire: the last wave out resets the queue for the next launch.
+    __threadfence();
+    if (lane == 0) {
+        const uint32_t done = atomicAdd(&a.counter[1], 1u);
+        if (done == a.total_waves - 1) {
+            atomicExch(&a.counter[0], 0u);
+            atomicExch(&a.counter[1], 0u);
+        }
+    }
+}
+
+using BatchFn = void (*)(BatchArgs);
+
+BatchFn pick_batch(int nb, int d) {
+    switch (nb) {
+    case 1: return d == 4 ? (BatchFn)k_batch<1, 8, 4> : (BatchFn)k_batch<1, 8, 8>;
+    case 2: return d == 4 ? (BatchFn)k_batch<2, 8, 4> : (BatchFn)k_batch<2, 8, 8>;
+    case 4: return (BatchFn)k_batch<4, 4, 8>;
+    default: return (BatchFn)k_batch<8, 2, 8>;
+    }
+}
+
+}  // namespace
+
+int batch_rows_per_item(int nb) { return nb <= 2 ? 8 : nb == 4 ? 4 : 2; }
+
+int batch_min_dwords(int nb) { return nb >= 4 ? 8 : 4; }
+
+hipError_t launch_batch(int nb, int dwords, const BatchArgs &args, uint32_t blocks,
+                        hipStream_t stream) {
+    hipLaunchKernelGGL(pick_batch(nb, dwords), dim3(blocks), dim3(BATCH_BLOCK), 0, stream, args);
+    return hipGetLastError();
+}
+
+uint32_t batch_block_threads() { return BATCH_BLOCK; }
+
+// Host: the work list for a set of stripes. Items are (stripe, chunk, row
+// group); `order` 0 keeps stripe order, 1 sorts by descending per-item cost
+// (longest-processing-time first) so the big items start early and the queue
+// drains evenly.
+std::vector<WorkItem> build_work_items(const std::vector<StripeDesc> &stripes, int nb, int dwords,
+                                       int order) {
+    const uint64_t chunk = 256u * (uint64_t)dwords;
+    const uint32_t rows = (uint32_t)batch_rows_per_item(nb);
+    std::vector<WorkItem> items;
+    for (uint32_t s = 0; s < stripes.size(); ++s) {
+        const StripeDesc &d = stripes[s];
+        const uint32_t nchunks = (uint32_t)((d.row_len + chunk - 1) / chunk);
+        const uint32_t ngroups = (d.m + rows - 1) / rows;
+        for (uint32_t c = 0; c < nchunks; ++c)
+            for (uint32_t g = 0; g < ngroups; ++g) items.push_back({s, c, g, 0});
+    }
+    if (order == 1) {
+        auto cost = [&](const WorkItem &w) {
+            const StripeDesc &d = stripes[w.stripe];
+            return (uint64_t)d.k * std::min<uint32_t>(rows, d.m - w.rowgroup * rows);
+        };
+        std::stable_sort(items.begin(), items.end(),
+                         [&](const WorkItem &x, const WorkItem &y) { return cost(x) > cost(y); });
+    }
+    return items;
+}
+
+}  // namespace fgf
+
+// ---------------------------------------------------------------------------
+// C ABI.
+// ---------------------------------------------------------------------------
+
+#include "plan.hpp"
+
+struct fgf_batch {
+    int field = 0;
+    int nb = 1;
+    int dwords = 8;
+    int order = 1;
+    uint32_t waves_per_cu = 16;
+    size_t nstripes = 0;
+    std::vector<fgf::StripeDesc> stripes;   // host copy (device pointers inside)
+    fgf::StripeDesc *d_stripes = nullptr;
+    fgf::WorkItem *d_items = nullptr;
+    size_t nitems = 0, items_cap = 0;
+    void *d_ptrs = nullptr;                  // all srcs/rows pointer arrays
+    uint32_t *d_counter = nullptr;
+    uint32_t blocks = 0;
+    uint64_t bytes = 0;                      // algorithmic bytes per run
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+int bfail(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+int bfail(const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    fgf::set_last_error(buf);
+    return FGF_EINVAL;
+}
+
+int bdev(const char *what, hipError_t e) {
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    fgf::set_last_error(m.c_str());
+    return FGF_EDEVICE;
+}
+
+void batch_release_graph(fgf_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+int batch_build_items(fgf_batch *b) {
+    std::vector<fgf::WorkItem> items = fgf::build_work_items(b->stripes, b->nb, b->dwords, b->order);
+    b->nitems = items.size();
+    if (b->nitems > b->items_cap) {
+        if (b->d_items) (void)hipFree(b->d_items);
+        b->d_items = nullptr;
+        hipError_t e = hipMalloc((void **)&b->d_items, std::max<size_t>(b->nitems, 1) * sizeof(fgf::WorkItem));
+        if (e != hipSuccess) return bdev("hipMalloc", e);
+        b->items_cap = b->nitems;
+    }
+    if (b->nitems) {
+        hipError_t e = hipMemcpy(b->d_items, items.data(), b->nitems * sizeof(fgf::WorkItem),
+                                 hipMemcpyHostToDevice);
+        if (e != hipSuccess) return bdev("hipMemcpy", e);
+    }
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const uint64_t waves_per_block = fgf::batch_block_threads() / 64;
+    const uint64_t want_waves = (uint64_t)cus * b->waves_per_cu;
+    const uint64_t waves = std::max<uint64_t>(1, std::min<uint64_t>(want_waves, b->nitems));
+    b->blocks = (uint32_t)((waves + waves_per_block - 1) / waves_per_block);
+    batch_release_graph(b);
+    return FGF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fgf_batch_create(fgf_field f, const fgf_stripe *stripes, size_t n, fgf_batch **out) {
+    using namespace fgf;
+    if (!valid_field(f)) return bfail("unknown field id %d", (int)f);
+    if (!out) return bfail("fgf_batch_create: output pointer is null");
+    const int nb = field_bytes(f);
+    size_t nptrs = 0;
+    for (size_t s = 0; s < n; ++s) {
+        const fgf_stripe &st = stripes[s];
+        const fgf_plan *p = st.plan;
+        if (!p) return bfail("batch stripe %zu: plan is null", s);
+        if (p->field != f)
+            return bfail("batch stripe %zu: plan is over %s, not %s", s, field_name(p->field), field_name(f));
+        if (p->rows == 0 || p->cols == 0)
+            return bfail("batch stripe %zu: plan dimensions (%zu, %zu) must be nonzero", s, p->rows, p->cols);
+        if (st.row_len % 16 != 0)
+            return bfail("batch stripe %zu: row_len %zu is not a multiple of 16 bytes", s, st.row_len);
+        for (size_t t = 0; t < p->rows; ++t)
+            if ((uintptr_t)st.srcs[t] & 15u)
+                return bfail("batch stripe %zu: source %zu is not 16-byte aligned", s, t);
+        for (size_t j = 0; j < p->cols; ++j)
+            if ((uintptr_t)st.rows[j] & 15u)
+                return bfail("batch stripe %zu: row %zu is not 16-byte aligned", s, j);
+        // Pairwise-disjoint destination rows (the scattered-row contract, src/ops.rs:909-921).
+        for (size_t a = 0; a < p->cols; ++a)
+            for (size_t c = a + 1; c < p->cols; ++c) {
+                const uintptr_t x = (uintptr_t)st.rows[a], y = (uintptr_t)st.rows[c];
+                if ((x < y ? y - x : x - y) < st.row_len)
+                    return bfail("batch stripe %zu: rows %zu and %zu overlap for %zu-byte rows", s, a, c, st.row_len);
+            }
+        nptrs += p->rows + p->cols;
+    }
+    auto *b = new (std::nothrow) fgf_batch;
+    if (!b) return bfail("out of host memory");
+    b->field = f;
+    b->nb = nb;
+    b->dwords = 8;
+    b->nstripes = n;
+    std::vector<const void *> ptrs;
+    ptrs.reserve(nptrs);
+    std::vector<size_t> offs(n);
+    for (size_t s = 0; s < n; ++s) {
+        offs[s] = ptrs.size();
+        for (size_t t = 0; t < stripes[s].plan->rows; ++t) ptrs.push_back(stripes[s].srcs[t]);
+        for (size_t j = 0; j < stripes[s].plan->cols; ++j) ptrs.push_back(stripes[s].rows[j]);
+    }
+    hipError_t e;
+    if ((e = hipMalloc(&b->d_ptrs, std::max<size_t>(ptrs.size(), 1) * sizeof(void *))) != hipSuccess ||
+        (e = hipMalloc((void **)&b->d_stripes, std::max<size_t>(n, 1) * sizeof(StripeDesc))) != hipSuccess ||
+        (e = hipMalloc((void **)&b->d_counter, 2 * sizeof(uint32_t))) != hipSuccess) {
+        fgf_batch_free(b);
+        return bdev("hipMalloc", e);
+    }
+    const void **dp = (const void **)b->d_ptrs;
+    b->stripes.resize(n);
+    const size_t e_dw = (size_t)nb * nb * TAB_DW;
+    for (size_t s = 0; s < n; ++s) {
+        const fgf_stripe &st = stripes[s];
+        const fgf_plan *p = st.plan;
+        StripeDesc &d = b->stripes[s];
+        d.srcs = (const uint8_t *const *)(dp + offs[s]);
+        d.rows = (uint8_t *const *)(dp + offs[s] + p->rows);
+        d.tab = p->d_tab;
+        d.row_len = st.row_len;
+        d.k = (uint32_t)p->rows;
+        d.m = (uint32_t)p->cols;
+        d.tab_stride_t = (uint32_t)(p->cols * e_dw);
+        d.tab_stride_j = (uint32_t)e_dw;
+        d.overwrite = st.overwrite ? 1u : 0u;
+        d.pad = 0;
+        b->bytes += (uint64_t)st.row_len * (p->rows + (st.overwrite ? 1u : 2u) * p->cols);
+    }
+    if ((e = hipMemcpy(b->d_ptrs, ptrs.data(), ptrs.size() * sizeof(void *), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(b->d_stripes, b->stripes.data(), n * sizeof(StripeDesc), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemset(b->d_counter, 0, 2 * sizeof(uint32_t))) != hipSuccess) {
+        fgf_batch_free(b);
+        return bdev("hipMemcpy", e);
+    }
+    int rc = batch_build_items(b);
+    if (rc) {
+        fgf_batch_free(b);
+        return rc;
+    }
+    *out = b;
+    return FGF_OK;
+}
+
+int fgf_batch_configure(fgf_batch *b, int dwords, int waves_per_cu, int order) {
+    if (!b) return bfail("fgf_batch_configure: batch is null");
+    if (dwords != 4 && dwords != 8) return bfail("fgf_batch_configure: dwords must be 4 or 8");
+    if (dwords < fgf::batch_min_dwords(b->nb))
+        return bfail("fgf_batch_configure: %d-byte elements need dwords >= %d", b->nb, fgf::batch_min_dwords(b->nb));
+    if (waves_per_cu < 1 || waves_per_cu > 32) return bfail("fgf_batch_configure: waves_per_cu must be in 1..32");
+    if (order != 0 && order != 1) return bfail("fgf_batch_configure: order must be 0 or 1");
+    b->dwords = dwords;
+    b->waves_per_cu = (uint32_t)waves_per_cu;
+    b->order = order;
+    return batch_build_items(b);
+}
+
+int fgf_batch_run(fgf_batch *b, void *stream) {
+    if (!b) return bfail("fgf_batch_run: batch is null");
+    if (b->nitems == 0) return FGF_OK;
+    fgf::BatchArgs a{b->d_stripes, b->d_items, (uint32_t)b->nitems,
+                     b->blocks * (fgf::batch_block_threads() / 64), b->d_counter};
+    hipError_t e = fgf::launch_batch(b->nb, b->dwords, a, b->blocks, (hipStream_t)stream);
+    return e == hipSuccess ? FGF_OK : bdev("batch launch", e);
+}
+
+int fgf_batch_run_graph(fgf_batch *b, void *stream) {
+    if (!b) return bfail("fgf_batch_run_graph: batch is null");
+    if (b->nitems == 0) return FGF_OK;
+    hipStream_t s = (hipStream_t)stream;
+    if (!b->exec) {
+        hipStream_t cap;
+        hipError_t e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+        if (e != hipSuccess) return bdev("hipStreamCreate", e);
+        e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            int rc = fgf_batch_run(b, cap);
+            hipGraph_t g = nullptr;
+            hipError_t e2 = hipStreamEndCapture(cap, &g);
+            if (rc != FGF_OK || e2 != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                (void)hipStreamDestroy(cap);
+                return rc != FGF_OK ? rc : bdev("hipStreamEndCapture", e2);
+            }
+            b->graph = g;
+            e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+        }
+        (void)hipStreamDestroy(cap);
+        if (e != hipSuccess) {
+            batch_release_graph(b);
+            return bdev("graph capture", e);
+        }
+    }
+    hipError_t e = hipGraphLaunch(b->exec, s);
+    return e == hipSuccess ? FGF_OK : bdev("hipGraphLaunch", e);
+}
+
+size_t fgf_batch_items(const fgf_batch *b) { return b ? b->nitems : 0; }
+
+uint64_t fgf_batch_bytes(const fgf_batch *b) { return b ? b->bytes : 0; }
+
+void fgf_batch_free(fgf_batch *b) {
+    if (!b) return;
+    batch_release_graph(b);
+    if (b->d_items) (void)hipFree(b->d_items);
+    if (b->d_stripes) (void)hipFree(b->d_stripes);
+    if (b->d_ptrs) (void)hipFree(b->d_ptrs);
+    if (b->d_counter) (void)hipFree(b->d_counter);
+    delete b;
+}
+
+}  // extern "C"

@@ -1,18 +1,10 @@
-// HIP kernels for fgf's packed vector operations on gfx950.
+// Per-call HIP kernels for fgf's packed vector operations on gfx950.
 //
 // Matrix family (dot_product_matrix, mul_add_matrix, mul_add_matrix_scattered,
 // scatter, gather, dot_product, mul_add, mul_into, mul_assign, add_assign):
-//   rows[j] (=|^=) sum_t c(t, j) * src[t]
-// Multiplying a packed element by a fixed coefficient is a B x B matrix over
-// the byte subfield (src/kernel/tables.rs:223-237 states the same fact for the
-// GF(2^16) tower). Each of its byte constants l is applied to four bytes at a
-// time with three v_perm_b32 lookups on 3-, 3- and 2-bit slices of every byte.
-// Elements of width B are transposed in registers into B byte planes of four
-// elements each, so every plane is multiplied by one constant per (p, q).
-// A wave keeps the M destination rows of its column tile in VGPRs while it
-// folds in all k sources, so every source byte crosses HBM once and every
-// destination byte is written once (the blocking of the reference's
-// x86 matrix_impl, src/kernel/x86/gf8.rs:2024-2067, at GPU tile sizes).
+// the register-tiled v_perm kernel of matrix_core.hpp over aligned 16-byte
+// pieces, plus a byte-wise generic kernel for heads, tails and unaligned
+// buffers.
 //
 // Elementwise family (mul_elementwise): products through discrete-log tables
 // of the byte field held in LDS, composed by the tower recursions.
@@ -23,6 +15,7 @@
 
 #include "field.hpp"
 #include "kernels.hpp"
+#include "matrix_core.hpp"
 
 namespace fgf {
 
@@ -52,84 +45,7 @@ void build_coeff_table(int field, uint64_t c, uint32_t *out) {
 
 namespace {
 
-// ---------------------------------------------------------------------------
-// Device building blocks.
-// ---------------------------------------------------------------------------
-
-__device__ __forceinline__ uint32_t vperm(uint32_t hi, uint32_t lo, uint32_t sel) {
-    return __builtin_amdgcn_perm(hi, lo, sel);
-}
-
-// Byte-plane transforms for a group of four elements held in B dwords.
-// Input: the group's bytes in memory order (element e at bytes e*B..e*B+B-1,
-// for B = 8 element e is dwords 2e, 2e+1). Output: plane p holds byte p of
-// elements 0..3.
-__device__ __forceinline__ void transpose4(uint32_t &a, uint32_t &b, uint32_t &c, uint32_t &d) {
-    const uint32_t t0 = vperm(b, a, 0x05010400u), t1 = vperm(b, a, 0x07030602u);
-    const uint32_t t2 = vperm(d, c, 0x05010400u), t3 = vperm(d, c, 0x07030602u);
-    a = vperm(t2, t0, 0x05040100u);
-    b = vperm(t2, t0, 0x07060302u);
-    c = vperm(t3, t1, 0x05040100u);
-    d = vperm(t3, t1, 0x07060302u);
-}
-
-template <int B>
-__device__ __forceinline__ void to_planes(const uint32_t *w, uint32_t *p) {
-    if constexpr (B == 1) {
-        p[0] = w[0];
-    } else if constexpr (B == 2) {
-        p[0] = vperm(w[1], w[0], 0x06040200u);
-        p[1] = vperm(w[1], w[0], 0x07050301u);
-    } else if constexpr (B == 4) {
-        p[0] = w[0]; p[1] = w[1]; p[2] = w[2]; p[3] = w[3];
-        transpose4(p[0], p[1], p[2], p[3]);
-    } else {
-        p[0] = w[0]; p[1] = w[2]; p[2] = w[4]; p[3] = w[6];
-        p[4] = w[1]; p[5] = w[3]; p[6] = w[5]; p[7] = w[7];
-        transpose4(p[0], p[1], p[2], p[3]);
-        transpose4(p[4], p[5], p[6], p[7]);
-    }
-}
-
-template <int B>
-__device__ __forceinline__ void from_planes(const uint32_t *p, uint32_t *w) {
-    if constexpr (B == 1) {
-        w[0] = p[0];
-    } else if constexpr (B == 2) {
-        w[0] = vperm(p[1], p[0], 0x05010400u);
-        w[1] = vperm(p[1], p[0], 0x07030602u);
-    } else if constexpr (B == 4) {
-        w[0] = p[0]; w[1] = p[1]; w[2] = p[2]; w[3] = p[3];
-        transpose4(w[0], w[1], w[2], w[3]);
-    } else {
-        uint32_t a = p[0], b = p[1], c = p[2], d = p[3];
-        transpose4(a, b, c, d);
-        w[0] = a; w[2] = b; w[4] = c; w[6] = d;
-        a = p[4]; b = p[5]; c = p[6]; d = p[7];
-        transpose4(a, b, c, d);
-        w[1] = a; w[3] = b; w[5] = c; w[7] = d;
-    }
-}
-
-struct Sel {
-    uint32_t s0, s1, s2;
-};
-
-__device__ __forceinline__ Sel selectors(uint32_t v) {
-    return {v & 0x07070707u, (v >> 3) & 0x07070707u, (v >> 6) & 0x03030303u};
-}
-
-// a ^ b ^ c in one v_bitop3_b32 (truth table 0x96).
-__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
-    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
-}
-
-// acc ^ l * v for the byte constant l whose tables start at e: three lookups
-// folded into the accumulator with two ops.
-__device__ __forceinline__ uint32_t cmul_acc(uint32_t acc, const uint32_t *__restrict__ e,
-                                             const Sel &s) {
-    return xor3(acc, vperm(e[1], e[0], s.s0), vperm(e[3], e[2], s.s1)) ^ vperm(e[4], e[4], s.s2);
-}
+using namespace core;
 
 struct MatrixArgs {
     const uint8_t *src[KMAX];
@@ -165,104 +81,18 @@ __device__ __forceinline__ bool map_block(const MatrixArgs &a, uint32_t &ct, uin
 }
 
 // Vector kernel: every pointer + col0 is 16-byte aligned and ncols is a
-// multiple of 16. A lane owns D dwords per source: 16-byte pieces at
-// tile + i*1024 + lane*16, so each load instruction covers 1 KiB contiguous.
+// multiple of 16.
 template <int B, int M, int D, bool OW>
 __global__ __launch_bounds__(BLOCK) void k_matrix(const MatrixArgs a) {
-    constexpr int P = D / 4;             // 16-byte pieces per lane
-    constexpr int G = D / B;             // groups of four elements per lane
-    constexpr uint32_t WAVE_BYTES = 256u * D;
-    static_assert(D % 4 == 0 && D % B == 0, "tile shape");
-
     uint32_t ct, rg;
     if (!map_block(a, ct, rg)) return;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint64_t tile = ((uint64_t)ct * (BLOCK / 64) + wave) * WAVE_BYTES;
+    const uint64_t tile = ((uint64_t)ct * (BLOCK / 64) + wave) * (256u * D);
     if (tile >= a.ncols) return;
     const uint32_t j0 = rg * M;
-    const uint32_t mrows = min((uint32_t)M, a.m - j0);
-
-    uint64_t off[P];
-    bool ok[P];
-#pragma unroll
-    for (int i = 0; i < P; ++i) {
-        const uint64_t rel = tile + (uint64_t)i * 1024u + lane * 16u;
-        ok[i] = rel < a.ncols;
-        off[i] = a.col0 + rel;
-    }
-
-    uint32_t acc[M][D];
-#pragma unroll
-    for (int j = 0; j < M; ++j) {
-        uint32_t w[D];
-#pragma unroll
-        for (int i = 0; i < P; ++i) {
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (!OW && j < (int)mrows && ok[i]) v = *(const uint4 *)(a.dst[j0 + j] + off[i]);
-            w[4 * i + 0] = v.x; w[4 * i + 1] = v.y; w[4 * i + 2] = v.z; w[4 * i + 3] = v.w;
-        }
-#pragma unroll
-        for (int g = 0; g < G; ++g) to_planes<B>(&w[g * B], &acc[j][g * B]);
-    }
-
-    uint4 cur[P], nxt[P];
-#pragma unroll
-    for (int i = 0; i < P; ++i)
-        cur[i] = ok[i] ? *(const uint4 *)(a.src[0] + off[i]) : make_uint4(0, 0, 0, 0);
-
-    const uint32_t *tab_rg = a.tab + (size_t)j0 * a.tab_stride_j;
-    for (uint32_t t = 0; t < a.k; ++t) {
-        if (t + 1 < a.k) {
-            const uint8_t *s = a.src[t + 1];
-#pragma unroll
-            for (int i = 0; i < P; ++i)
-                nxt[i] = ok[i] ? *(const uint4 *)(s + off[i]) : make_uint4(0, 0, 0, 0);
-        }
-        uint32_t w[D], pl[D];
-#pragma unroll
-        for (int i = 0; i < P; ++i) {
-            w[4 * i + 0] = cur[i].x; w[4 * i + 1] = cur[i].y;
-            w[4 * i + 2] = cur[i].z; w[4 * i + 3] = cur[i].w;
-        }
-#pragma unroll
-        for (int g = 0; g < G; ++g) to_planes<B>(&w[g * B], &pl[g * B]);
-        Sel sel[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) sel[d] = selectors(pl[d]);
-
-        const uint32_t *tt = tab_rg + (size_t)t * a.tab_stride_t;
-#pragma unroll
-        for (int j = 0; j < M; ++j) {
-            if (j < (int)mrows) {
-                const uint32_t *tj = tt + (size_t)j * a.tab_stride_j;
-#pragma unroll
-                for (int p = 0; p < B; ++p) {
-#pragma unroll
-                    for (int q = 0; q < B; ++q) {
-                        const uint32_t *e = tj + (p * B + q) * TAB_DW;
-#pragma unroll
-                        for (int g = 0; g < G; ++g) acc[j][g * B + p] = cmul_acc(acc[j][g * B + p], e, sel[g * B + q]);
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < P; ++i) cur[i] = nxt[i];
-    }
-
-#pragma unroll
-    for (int j = 0; j < M; ++j) {
-        if (j < (int)mrows) {
-            uint32_t w[D];
-#pragma unroll
-            for (int g = 0; g < G; ++g) from_planes<B>(&acc[j][g * B], &w[g * B]);
-#pragma unroll
-            for (int i = 0; i < P; ++i)
-                if (ok[i])
-                    *(uint4 *)(a.dst[j0 + j] + off[i]) =
-                        make_uint4(w[4 * i + 0], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
-        }
-    }
+    matrix_tile<B, M, D>(a.src, a.dst + j0, a.tab + (size_t)j0 * a.tab_stride_j, a.tab_stride_t,
+                         a.tab_stride_j, a.k, min((uint32_t)M, a.m - j0), OW, a.col0, a.ncols, tile,
+                         lane);
 }
 
 // Generic kernel for heads, tails and unaligned buffers: one thread per group

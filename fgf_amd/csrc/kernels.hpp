@@ -47,4 +47,7 @@ hipError_t launch_fill_zero(uint8_t *dst, size_t len, hipStream_t stream);
 // Host-side table builder: writes the B*B*TAB_DW dwords of coefficient c.
 void build_coeff_table(int field, uint64_t c, uint32_t *out);
 
+// Sets the thread-local text returned by fgf_last_error().
+void set_last_error(const char *msg);
+
 }  // namespace fgf

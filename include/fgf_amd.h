@@ -72,6 +72,12 @@ const char *fgf_field_name(fgf_field field);
 int fgf_elem_mul(fgf_field field, uint64_t a, uint64_t b, uint64_t *out);
 int fgf_elem_inv(fgf_field field, uint64_t a, uint64_t *out);
 
+/* Host Gauss-Jordan inverse of an n x n row-major matrix of packed elements
+ * (n*n*B bytes in and out). Returns FGF_EINVAL if singular. The reference
+ * has no matrix inversion (src/lib.rs:115-116, codecs live above it); decode
+ * matrices for reconstruction are built with it. */
+int fgf_matrix_invert(fgf_field field, size_t n, const void *in, void *out);
+
 /* ------------------------------------------------------------------------
  * Prepared coefficients: Coeff<F> (src/ops.rs:53-87) and Plan<F>
  * (src/ops.rs:165-298). A Coeff is a plan of one coefficient; a CoeffRef
@@ -191,6 +197,40 @@ int fgf_mul_elementwise(fgf_field field, uint8_t *dst, size_t dst_len, const uin
  * are private, src/kernel/tables.rs); it lets a CPU model check the table
  * math. `out` holds fgf_field_bytes(field)^2 * 5 dwords. */
 int fgf_coeff_table(fgf_field field, uint64_t coeff, uint32_t *out);
+
+/* ------------------------------------------------------------------------
+ * Batched launch. No counterpart in the reference (its calls are one stripe
+ * at a time): many independent matrix operations, each with its own plan,
+ * geometry and overwrite/accumulate mode, executed by one persistent kernel
+ * from a device-resident descriptor queue. Each stripe computes exactly what
+ * fgf_dot_product_matrix_with (overwrite = 1) or fgf_mul_add_matrix_with
+ * (overwrite = 0) computes for the same arguments. Requirements beyond those
+ * calls: row_len a multiple of 16, every pointer 16-byte aligned, plan
+ * dimensions (k, m) both nonzero. Plans and buffers must outlive the batch;
+ * distinct stripes must not write overlapping rows.
+ * ---------------------------------------------------------------------- */
+typedef struct fgf_batch fgf_batch;
+typedef struct fgf_stripe {
+    const fgf_plan *plan;         /* (k, m) coefficient plan */
+    const uint8_t *const *srcs;   /* host array of k device pointers, row_len bytes each */
+    uint8_t *const *rows;         /* host array of m device pointers, pairwise disjoint */
+    size_t row_len;
+    int overwrite;                /* 1: rows = sum; 0: rows ^= sum */
+} fgf_stripe;
+/* Validate the stripes and upload the descriptor queue to the current device. */
+int fgf_batch_create(fgf_field field, const fgf_stripe *stripes, size_t nstripes,
+                     fgf_batch **out);
+/* Tile shape and launch shape: dwords per lane per source (4 or 8), resident
+ * waves per CU (1..32), work order (0 = stripe order, 1 = largest items first). */
+int fgf_batch_configure(fgf_batch *batch, int dwords, int waves_per_cu, int order);
+/* One persistent launch over every stripe, on `stream`. */
+int fgf_batch_run(fgf_batch *batch, void *stream);
+/* The same launch through a hipGraph captured on first use and replayed after. */
+int fgf_batch_run_graph(fgf_batch *batch, void *stream);
+size_t fgf_batch_items(const fgf_batch *batch);
+/* Algorithmic HBM bytes of one run: sum of (k + m) * L (overwrite) or (k + 2m) * L. */
+uint64_t fgf_batch_bytes(const fgf_batch *batch);
+void fgf_batch_free(fgf_batch *batch);
 
 /* ------------------------------------------------------------------------
  * Element packing (src/ops.rs:963-1017), host memory only.

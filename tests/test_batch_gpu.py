@@ -1,0 +1,101 @@
+"""Batched persistent launch: every stripe's output equals the per-call
+kernel's output for the same arguments (and the oracle on sampled stripes),
+for plain launches, graph replays and every tile configuration; reconstruct
+stripes give back the erased shards.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle_lib as ol
+from fgf_amd import Gf8D, Gf16, Panic, ops, workloads
+from fgf_amd.batch import Batch, Stripe
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    if not torch.cuda.is_available():
+        pytest.skip("no gfx950 device")
+
+
+def build(field, specs, pool):
+    plans, stripes = {}, []
+    for sp in specs:
+        key = (sp.k, len(sp.out_rows), tuple(sp.plan_values))
+        if key not in plans:
+            plans[key] = ops.Plan.matrix(field, sp.k, len(sp.out_rows), sp.plan_values)
+        srcs, rows = workloads.stripe_buffers(pool, sp)
+        stripes.append(Stripe(plans[key], srcs, rows, sp.row_len, True))
+    return Batch(field, stripes), plans
+
+
+def per_call(field, specs, pool, plans):
+    for sp in specs:
+        srcs, rows = workloads.stripe_buffers(pool, sp)
+        key = (sp.k, len(sp.out_rows), tuple(sp.plan_values))
+        out = torch.empty(len(rows) * sp.row_len, dtype=torch.uint8, device="cuda")
+        ops.dot_product_matrix_with(field, out, sp.row_len, len(rows), plans[key], srcs)
+        yield sp, out
+
+
+@pytest.mark.parametrize("field", [Gf8D, Gf16], ids=["gf8d", "gf16"])
+def test_batch_matches_per_call_and_restores_erasures(field):
+    specs = workloads.draw_uneven_mix(field, 96, seed=11, lmin=1024, lmax=64 << 10)
+    pool, saved = workloads.materialize(field, specs, seed=12)
+    batch, plans = build(field, specs, pool)
+    want = [out.clone() for _, out in per_call(field, specs, pool, plans)]
+    for cfg in [(8, 16, 1), (4, 8, 0), (8, 4, 0), (4, 32, 1)]:
+        if field.BYTES == 1 or cfg[0] >= 4:
+            batch.configure(*cfg)
+        for runner in (batch.run, batch.run_graph, batch.run_graph):
+            for i, sp in enumerate(specs):               # clobber the outputs first
+                for r in workloads.stripe_buffers(pool, sp)[1]:
+                    r.fill_(0xA5)
+            runner()
+            torch.cuda.synchronize()
+            for i, sp in enumerate(specs):
+                got = torch.cat(workloads.stripe_buffers(pool, sp)[1])
+                assert torch.equal(got, want[i]), (cfg, runner.__name__, i, sp.kind)
+                if sp.kind == "reconstruct":
+                    assert torch.equal(got, saved[i]), (cfg, i)
+    # sampled oracle check of the per-call reference itself
+    name = "gf8d" if field is Gf8D else "gf16"
+    for i in range(0, len(specs), 17):
+        sp = specs[i]
+        srcs = [s.cpu().numpy() for s in workloads.stripe_buffers(pool, sp)[0]]
+        exp = np.zeros(len(sp.out_rows) * sp.row_len, np.uint8)
+        ol.mul_add_matrix(name, exp, sp.row_len, len(sp.out_rows), sp.plan_values, srcs, True)
+        assert np.array_equal(want[i].cpu().numpy(), exp)
+
+
+def test_batch_accumulate_mode_and_wide_fields():
+    from fgf_amd import FanPaar64, Gf64
+    for field in (Gf64, FanPaar64):
+        k, m, L = 5, 3, 4096 + 512
+        g = np.random.default_rng(3)
+        srcs_h = [g.integers(0, 256, L, dtype=np.uint8) for _ in range(k)]
+        vals = [int(v) for v in g.integers(0, 1 << 62, k * m, dtype=np.uint64)]
+        plan = ops.Plan.matrix(field, k, m, vals)
+        srcs = [torch.from_numpy(s).cuda() for s in srcs_h]
+        init = g.integers(0, 256, m * L, dtype=np.uint8)
+        rows = torch.from_numpy(init.copy()).cuda()
+        b = Batch(field, [Stripe(plan, srcs, [rows[j * L:(j + 1) * L] for j in range(m)], L, False)])
+        b.run()
+        torch.cuda.synchronize()
+        name = "gf64" if field is Gf64 else "fp64"
+        exp = init.copy()
+        ol.mul_add_matrix(name, exp, L, m, vals, srcs_h, False)
+        assert np.array_equal(rows.cpu().numpy(), exp)
+
+
+def test_batch_rejects_misaligned_and_overlapping_rows():
+    plan = ops.Plan.matrix(Gf8D, 1, 2, [3, 5])
+    buf = torch.zeros(8192, dtype=torch.uint8, device="cuda")
+    with pytest.raises(Panic, match="not a multiple of 16"):
+        Batch(Gf8D, [Stripe(plan, [buf[:100]], [buf[1000:1100], buf[2000:2100]], 100)])
+    with pytest.raises(Panic, match="not 16-byte aligned"):
+        Batch(Gf8D, [Stripe(plan, [buf[1:65]], [buf[1024:1088], buf[2048:2112]], 64)])
+    with pytest.raises(Panic, match="overlap"):
+        Batch(Gf8D, [Stripe(plan, [buf[:64]], [buf[1024:1088], buf[1056:1120]], 64)])
