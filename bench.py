@@ -95,6 +95,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--row-len", type=int, default=L_HEADLINE)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="after timing, run one 1 GiB device copy (a plain copy kernel of known "
+                         "size, for scaling HBM traffic counters in a --pmc profile)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -170,6 +173,13 @@ def main() -> int:
                      "traffic": None, "kernel_avg_ms": round(avg_ms, 5),
                      "kernel": "k_matrix<B=1,M=4,D=8,overwrite>"},
     }
+    if args.calib_copy:
+        del rows
+        a = src.view(-1)[: 1 << 30]
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print("[bench] calibration copy: 1 GiB read + 1 GiB written", file=sys.stderr, flush=True)
     if not args.no_cpu:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline()
