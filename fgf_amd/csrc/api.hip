@@ -12,7 +12,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/fgf_amd.h"
+#include "fgf_amd.h"
 #include "field.hpp"
 #include "kernels.hpp"
 #include "plan.hpp"

@@ -20,7 +20,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/fgf_amd.h"
+#include "fgf_amd.h"
 #include "batch.hpp"
 #include "field.hpp"
 #include "matrix_core.hpp"
@@ -29,6 +29,21 @@ namespace fgf {
 namespace {
 
 constexpr int BATCH_BLOCK = 256;
+
+// Runs the tile with the compile-time row count equal to `rows` (1..M).
+template <int B, int D, int... Ms>
+__device__ __forceinline__ void tile_rows(uint32_t rows, const StripeDesc &s, uint32_t j0,
+                                          uint64_t tile, uint32_t lane,
+                                          std::integer_sequence<int, Ms...>) {
+    ((rows == (uint32_t)(Ms + 1)
+          ? (core::matrix_tile<B, Ms + 1, D>(s.srcs, s.rows + j0,
+                                             s.tab + (size_t)j0 * s.tab_stride_j, s.tab_stride_t,
+                                             s.tab_stride_j, s.k, s.overwrite != 0, 0, s.row_len,
+                                             tile, lane),
+             0)
+          : 0),
+     ...);
+}
 
 template <int B, int M, int D>
 __global__ __launch_bounds__(BATCH_BLOCK) void k_batch(const BatchArgs a) {
@@ -43,9 +58,8 @@ __global__ __launch_bounds__(BATCH_BLOCK) void k_batch(const BatchArgs a) {
         const StripeDesc s = a.stripes[w.stripe];
         const uint32_t j0 = w.rowgroup * M;
         const uint32_t mrows = min((uint32_t)M, s.m - j0);
-        core::matrix_tile<B, M, D>(s.srcs, s.rows + j0, s.tab + (size_t)j0 * s.tab_stride_j,
-                                   s.tab_stride_t, s.tab_stride_j, s.k, mrows, s.overwrite != 0, 0,
-                                   s.row_len, (uint64_t)w.chunk * CHUNK, lane);
+        tile_rows<B, D>(mrows, s, j0, (uint64_t)w.chunk * CHUNK, lane,
+                        std::make_integer_sequence<int, M>{});
     }
     // Retire: the last wave out resets the queue for the next launch.
     __threadfence();

@@ -77,6 +77,10 @@ __device__ __forceinline__ bool map_block(const MatrixArgs &a, uint32_t &ct, uin
         rg = rest % a.ngroups;
         ct = (rest / a.ngroups) * 8u + lo;
     }
+    // The division above is expanded with VALU ops; pin the results to SGPRs
+    // so every table and pointer address derived from them stays scalar.
+    ct = __builtin_amdgcn_readfirstlane(ct);
+    rg = __builtin_amdgcn_readfirstlane(rg);
     return ct < a.ncoltiles;
 }
 
@@ -86,13 +90,13 @@ template <int B, int M, int D, bool OW>
 __global__ __launch_bounds__(BLOCK) void k_matrix(const MatrixArgs a) {
     uint32_t ct, rg;
     if (!map_block(a, ct, rg)) return;
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint64_t tile = ((uint64_t)ct * (BLOCK / 64) + wave) * (256u * D);
     if (tile >= a.ncols) return;
     const uint32_t j0 = rg * M;
     matrix_tile<B, M, D>(a.src, a.dst + j0, a.tab + (size_t)j0 * a.tab_stride_j, a.tab_stride_t,
-                         a.tab_stride_j, a.k, min((uint32_t)M, a.m - j0), OW, a.col0, a.ncols, tile,
-                         lane);
+                         a.tab_stride_j, a.k, OW, a.col0, a.ncols, tile, lane);
 }
 
 // Generic kernel for heads, tails and unaligned buffers: one thread per group
@@ -143,7 +147,10 @@ using MatrixFn = void (*)(MatrixArgs);
 
 // Rows per block for each element width, bounded by the accumulator budget.
 constexpr int max_rows(int nb) { return nb == 1 ? 16 : nb == 2 ? 8 : nb == 4 ? 4 : 2; }
-constexpr int dwords_for(int nb, int m) { return (nb == 1 && m > 8) ? 4 : 8; }
+// Byte fields: 16 bytes per lane per source (1 KiB per wave per stream), the
+// granularity at which the encode mix streams fastest; wider elements keep
+// 32 bytes so a lane holds whole 4-element groups.
+constexpr int dwords_for(int nb, int m) { return nb == 1 ? 4 : 8; }
 
 template <int B, int M>
 MatrixFn pick_m(bool ow) {
@@ -190,7 +197,16 @@ hipError_t launch_segment(const MatrixJob &job, MatrixArgs a, size_t k0, size_t 
     a.ncols = ncols;
     const int nb = job.nb;
     if (vector) {
+        // Every block handles exactly mt rows; a ragged last row group gets a
+        // launch of its own with its own compile-time row count.
         const int mt = std::min<int>((int)mn, max_rows(nb));
+        const size_t full = (mn / mt) * mt;
+        if (full < mn) {
+            hipError_t e = launch_segment(job, a, k0, kn, j0 + full, mn - full, col0, ncols, vector, ow);
+            if (e != hipSuccess) return e;
+            mn = full;
+            a.m = (uint32_t)mn;
+        }
         const int d = dwords_for(nb, mt);
         const uint64_t block_bytes = (uint64_t)BLOCK / 64 * 256u * d;
         a.ngroups = (uint32_t)((mn + mt - 1) / mt);

@@ -98,29 +98,185 @@ __device__ __forceinline__ uint32_t cmul_acc(uint32_t acc, const uint32_t *__res
     return xor3(acc, vperm(e[1], e[0], s.s0), vperm(e[3], e[2], s.s1)) ^ vperm(e[4], e[4], s.s2);
 }
 
-// One wave's tile: D dwords per lane per source in 16-byte pieces at
-// col0 + tile + i*1024 + lane*16 (each load instruction covers 1 KiB
-// contiguous). Pieces at or beyond `ncols` (relative to col0) are skipped.
-// All pointers + col0 must be 16-byte aligned. Rows j < mrows (<= M) are
-// processed; `ow` selects overwrite (zero seed) or accumulate (seed from dst).
+// XOR-folds a stream of lookup results into an accumulator two at a time
+// with v_bitop3, so n lookups cost ceil(n / 2) folding ops. All calls are
+// unrolled at compile time; `pending` carries an odd lookup to the next add.
+struct Fold {
+    uint32_t acc;
+    uint32_t pending;
+    bool has;
+    __device__ __forceinline__ void add(uint32_t v) {
+        if (has) {
+            acc = xor3(acc, pending, v);
+            has = false;
+        } else {
+            pending = v;
+            has = true;
+        }
+    }
+    __device__ __forceinline__ uint32_t done() { return has ? (acc ^ pending) : acc; }
+};
+
+// Adds the three lookups of l * v (tables at e) to a fold.
+__device__ __forceinline__ void cmul_fold(Fold &f, const uint32_t *__restrict__ e, const Sel &s) {
+    f.add(vperm(e[1], e[0], s.s0));
+    f.add(vperm(e[3], e[2], s.s1));
+    f.add(vperm(e[4], e[4], s.s2));
+}
+
+// Per-lane geometry of one wave's tile: D dwords per lane per source in
+// 16-byte pieces at base + lo[i], lo[i] = i*1024 + lane*16, so each load
+// instruction covers 1 KiB contiguous. `base` (= col0 + tile) is uniform;
+// the lane offsets are the same for every source and destination row, so
+// every access is an SGPR base plus a fixed 32-bit VGPR offset.
+template <int P>
+struct TileLanes {
+    uint32_t lo[P];
+    bool ok[P];
+};
+
+// Streams are touched once, so loads and stores are non-temporal: on gfx950
+// this lets the 72/28 read/write mix of an encode run at about 6 TB/s where
+// cached accesses to the same addresses saturate near 5.4 TB/s.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void load16(uint32_t *x, const uint8_t *p) {
+    const u32x4 v = __builtin_nontemporal_load((const u32x4 *)p);
+    x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
+}
+
+__device__ __forceinline__ void store16(uint8_t *p, const uint32_t *w) {
+    const u32x4 v = {w[0], w[1], w[2], w[3]};
+    __builtin_nontemporal_store(v, (u32x4 *)p);
+}
+
+template <int D, typename SrcPtrs>
+__device__ __forceinline__ void load_source(uint32_t (&x)[D], SrcPtrs src, uint32_t t, uint64_t base,
+                                            const TileLanes<D / 4> &tl) {
+    const uint8_t *s = src[t] + base;
+#pragma unroll
+    for (int i = 0; i < D / 4; ++i) {
+        x[4 * i + 0] = x[4 * i + 1] = x[4 * i + 2] = x[4 * i + 3] = 0;
+        if (tl.ok[i]) load16(&x[4 * i], s + tl.lo[i]);
+    }
+}
+
+// Folds S (1 or 2) sources (dwords x0, x1) into the row accumulators. The
+// table of source s, row j, entry (p, q) starts at
+// tt + s*stride_t + j*stride_j + (p*B + q)*TAB_DW. Each (row, plane) folds
+// all of its 3*B*S lookups with v_bitop3. Two loop orders:
+//  - dword-outer: one dword's selectors live, every table of the step held
+//    in SGPRs (S*M*5 dwords) -- byte fields with few rows;
+//  - row-outer: all selectors of the step live, one row's tables at a time.
+template <int B, int M>
+constexpr bool dword_outer() { return B == 1 && M <= 6; }
+
+template <int B, int M, int D, int S>
+__device__ __forceinline__ void consume(uint32_t (&acc)[M][D], const uint32_t *x0,
+                                        const uint32_t *x1, const uint32_t *__restrict__ tt,
+                                        uint32_t stride_t, uint32_t stride_j) {
+    constexpr int G = D / B;
+    const uint32_t *xs[2] = {x0, x1};
+    if constexpr (dword_outer<B, M>()) {
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            Sel sel[S][B];
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                uint32_t pl[B];
+                to_planes<B>(&xs[s][g * B], pl);
+#pragma unroll
+                for (int q = 0; q < B; ++q) sel[s][q] = selectors(pl[q]);
+            }
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+#pragma unroll
+                for (int p = 0; p < B; ++p) {
+                    Fold f{acc[j][g * B + p], 0u, false};
+#pragma unroll
+                    for (int s = 0; s < S; ++s)
+#pragma unroll
+                        for (int q = 0; q < B; ++q)
+                            cmul_fold(f, tt + s * stride_t + j * stride_j + (p * B + q) * TAB_DW,
+                                      sel[s][q]);
+                    acc[j][g * B + p] = f.done();
+                }
+            }
+        }
+    } else {
+        Sel sel[S][D];
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                uint32_t pl[B];
+                to_planes<B>(&xs[s][g * B], pl);
+#pragma unroll
+                for (int q = 0; q < B; ++q) sel[s][g * B + q] = selectors(pl[q]);
+            }
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+#pragma unroll
+            for (int p = 0; p < B; ++p) {
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+                    Fold f{acc[j][g * B + p], 0u, false};
+#pragma unroll
+                    for (int s = 0; s < S; ++s)
+#pragma unroll
+                        for (int q = 0; q < B; ++q)
+                            cmul_fold(f, tt + s * stride_t + j * stride_j + (p * B + q) * TAB_DW,
+                                      sel[s][g * B + q]);
+                    acc[j][g * B + p] = f.done();
+                }
+            }
+        }
+    }
+}
+
+// Folds the sources t (and t + 1 when present) held in x.
+template <int B, int M, int D>
+__device__ __forceinline__ void consume_step(uint32_t (&acc)[M][D], const uint32_t (&x)[2][D],
+                                             const uint32_t *__restrict__ tab, uint32_t t,
+                                             uint32_t k, uint32_t stride_t, uint32_t stride_j) {
+    const uint32_t *tt = tab + __builtin_amdgcn_readfirstlane(t * stride_t);
+    if (t + 1 < k) {
+        if constexpr (B == 1) {
+            consume<B, M, D, 2>(acc, x[0], x[1], tt, stride_t, stride_j);
+        } else {
+            consume<B, M, D, 1>(acc, x[0], x[0], tt, stride_t, stride_j);
+            consume<B, M, D, 1>(acc, x[1], x[1], tt + stride_t, stride_t, stride_j);
+        }
+    } else {
+        consume<B, M, D, 1>(acc, x[0], x[0], tt, stride_t, stride_j);
+    }
+}
+
+// One wave's tile of rows[j] (=|^=) sum_t c(t, j) * src[t], for columns
+// [col0 + tile, col0 + tile + 256*D) clipped to col0 + ncols. All pointers
+// + col0 must be 16-byte aligned and ncols a multiple of 16. Exactly M rows
+// are processed (a compile-time count keeps every table load hoistable out
+// of the dword loop); `ow` selects overwrite (zero seed) or accumulate
+// (seed from the destination). Sources are consumed in pairs from two
+// ping-pong register buffers: the next pair's loads are in flight while the
+// current pair is multiplied.
 template <int B, int M, int D, typename SrcPtrs, typename DstPtrs>
 __device__ __forceinline__ void matrix_tile(SrcPtrs src, DstPtrs dst,
                                             const uint32_t *__restrict__ tab, uint32_t stride_t,
-                                            uint32_t stride_j, uint32_t k, uint32_t mrows, bool ow,
+                                            uint32_t stride_j, uint32_t k, bool ow,
                                             uint64_t col0, uint64_t ncols, uint64_t tile,
                                             uint32_t lane) {
     constexpr int P = D / 4;
     constexpr int G = D / B;
     static_assert(D % 4 == 0 && D % B == 0, "tile shape");
 
-    uint64_t off[P];
-    bool ok[P];
+    TileLanes<P> tl;
 #pragma unroll
     for (int i = 0; i < P; ++i) {
-        const uint64_t rel = tile + (uint64_t)i * 1024u + lane * 16u;
-        ok[i] = rel < ncols;
-        off[i] = col0 + rel;
+        tl.lo[i] = (uint32_t)i * 1024u + lane * 16u;
+        tl.ok[i] = tile + tl.lo[i] < ncols;
     }
+    const uint64_t base = col0 + tile;
 
     uint32_t acc[M][D];
 #pragma unroll
@@ -128,71 +284,37 @@ __device__ __forceinline__ void matrix_tile(SrcPtrs src, DstPtrs dst,
         uint32_t w[D];
 #pragma unroll
         for (int i = 0; i < P; ++i) {
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (!ow && j < (int)mrows && ok[i]) v = *(const uint4 *)(dst[j] + off[i]);
-            w[4 * i + 0] = v.x; w[4 * i + 1] = v.y; w[4 * i + 2] = v.z; w[4 * i + 3] = v.w;
+            w[4 * i + 0] = w[4 * i + 1] = w[4 * i + 2] = w[4 * i + 3] = 0;
+            if (!ow && tl.ok[i]) load16(&w[4 * i], dst[j] + base + tl.lo[i]);
         }
 #pragma unroll
         for (int g = 0; g < G; ++g) to_planes<B>(&w[g * B], &acc[j][g * B]);
     }
 
-    uint4 cur[P], nxt[P];
-#pragma unroll
-    for (int i = 0; i < P; ++i)
-        cur[i] = ok[i] ? *(const uint4 *)(src[0] + off[i]) : make_uint4(0, 0, 0, 0);
-
-    for (uint32_t t = 0; t < k; ++t) {
-        if (t + 1 < k) {
-            const uint8_t *s = src[t + 1];
-#pragma unroll
-            for (int i = 0; i < P; ++i)
-                nxt[i] = ok[i] ? *(const uint4 *)(s + off[i]) : make_uint4(0, 0, 0, 0);
-        }
-        uint32_t w[D], pl[D];
-#pragma unroll
-        for (int i = 0; i < P; ++i) {
-            w[4 * i + 0] = cur[i].x; w[4 * i + 1] = cur[i].y;
-            w[4 * i + 2] = cur[i].z; w[4 * i + 3] = cur[i].w;
-        }
-#pragma unroll
-        for (int g = 0; g < G; ++g) to_planes<B>(&w[g * B], &pl[g * B]);
-        Sel sel[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) sel[d] = selectors(pl[d]);
-
-        const uint32_t *tt = tab + (size_t)t * stride_t;
-#pragma unroll
-        for (int j = 0; j < M; ++j) {
-            if (j < (int)mrows) {
-                const uint32_t *tj = tt + (size_t)j * stride_j;
-#pragma unroll
-                for (int p = 0; p < B; ++p) {
-#pragma unroll
-                    for (int q = 0; q < B; ++q) {
-                        const uint32_t *e = tj + (p * B + q) * TAB_DW;
-#pragma unroll
-                        for (int g = 0; g < G; ++g)
-                            acc[j][g * B + p] = cmul_acc(acc[j][g * B + p], e, sel[g * B + q]);
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < P; ++i) cur[i] = nxt[i];
+    uint32_t xa[2][D], xb[2][D];
+    load_source<D>(xa[0], src, 0, base, tl);
+    if (k > 1) load_source<D>(xa[1], src, 1, base, tl);
+    for (uint32_t t = 0;;) {
+        if (t + 2 < k) load_source<D>(xb[0], src, t + 2, base, tl);
+        if (t + 3 < k) load_source<D>(xb[1], src, t + 3, base, tl);
+        consume_step<B, M, D>(acc, xa, tab, t, k, stride_t, stride_j);
+        t += 2;
+        if (t >= k) break;
+        if (t + 2 < k) load_source<D>(xa[0], src, t + 2, base, tl);
+        if (t + 3 < k) load_source<D>(xa[1], src, t + 3, base, tl);
+        consume_step<B, M, D>(acc, xb, tab, t, k, stride_t, stride_j);
+        t += 2;
+        if (t >= k) break;
     }
 
 #pragma unroll
     for (int j = 0; j < M; ++j) {
-        if (j < (int)mrows) {
-            uint32_t w[D];
+        uint32_t w[D];
 #pragma unroll
-            for (int g = 0; g < G; ++g) from_planes<B>(&acc[j][g * B], &w[g * B]);
+        for (int g = 0; g < G; ++g) from_planes<B>(&acc[j][g * B], &w[g * B]);
 #pragma unroll
-            for (int i = 0; i < P; ++i)
-                if (ok[i])
-                    *(uint4 *)(dst[j] + off[i]) =
-                        make_uint4(w[4 * i + 0], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
-        }
+        for (int i = 0; i < P; ++i)
+            if (tl.ok[i]) store16(dst[j] + base + tl.lo[i], &w[4 * i]);
     }
 }
 
