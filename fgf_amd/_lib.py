@@ -67,9 +67,10 @@ PROTOTYPES = {
     "fgf_mul_elementwise": (c_int, [c_int, c_u8p, c_size, c_u8p, c_size, c_u8p, c_size, c_vp]),
     "fgf_coeff_table": (c_int, [c_int, c_u64, ctypes.POINTER(ctypes.c_uint32)]),
     "fgf_batch_create": (c_int, [c_int, c_vp, c_size, ctypes.POINTER(c_vp)]),
-    "fgf_batch_configure": (c_int, [c_vp, c_int, c_int, c_int]),
+    "fgf_batch_configure": (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int]),
     "fgf_batch_run": (c_int, [c_vp, c_vp]),
     "fgf_batch_run_graph": (c_int, [c_vp, c_vp]),
+    "fgf_batch_run_per_call": (c_int, [c_vp, c_vp]),
     "fgf_batch_items": (c_size, [c_vp]),
     "fgf_batch_bytes": (ctypes.c_uint64, [c_vp]),
     "fgf_batch_free": (None, [c_vp]),

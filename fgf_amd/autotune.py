@@ -1,0 +1,110 @@
+"""Autotuner for the batched launch.
+
+Picks the tile and launch shape of `Batch` — dwords per lane (bytes per lane =
+4 * dwords), destination rows per wave, wave tiles per work item, resident
+waves per CU and work order — by timing candidates on the batch itself, and
+persists the winner per (GPU arch, field, shape bucket) in a JSON file
+($FGF_AUTOTUNE_CACHE, default ~/.cache/fgf_amd/autotune.json). A cached
+entry is reused without timing.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+from statistics import median
+
+CACHE_ENV = "FGF_AUTOTUNE_CACHE"
+
+
+def cache_path() -> str:
+    return os.environ.get(CACHE_ENV) or os.path.join(os.path.expanduser("~"), ".cache", "fgf_amd",
+                                                     "autotune.json")
+
+
+def _load() -> dict:
+    try:
+        with open(cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _save(db: dict) -> None:
+    p = cache_path()
+    try:
+        os.makedirs(os.path.dirname(p), exist_ok=True)
+        tmp = p + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(db, f, indent=1, sort_keys=True)
+        os.replace(tmp, p)
+    except OSError:
+        pass  # a read-only home only costs a re-tune next time
+
+
+def shape_bucket(batch) -> str:
+    """Coarse description of a batch: stripe count, mean k, mean m, mean row length."""
+    n = len(batch.stripes)
+    ks = [s.plan.dimensions()[0] for s in batch.stripes] or [0]
+    ms = [s.plan.dimensions()[1] for s in batch.stripes] or [0]
+    ls = [s.row_len for s in batch.stripes] or [1]
+    return "n{}-k{}-m{}-L{}".format(int(math.log2(max(n, 1))), round(sum(ks) / len(ks)),
+                                    round(sum(ms) / len(ms)),
+                                    int(math.log2(max(sum(ls) / len(ls), 1))))
+
+
+def candidates(nb: int):
+    rows = [4, 8] if nb <= 2 else [4 if nb == 4 else 2]
+    dws = [4, 8] if nb <= 2 else [8]
+    return rows, dws
+
+
+def _time(batch, reps: int) -> float:
+    import torch
+    s = torch.cuda.current_stream()
+    batch.run()
+    torch.cuda.synchronize()
+    out = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(s)
+        batch.run()
+        b.record(s)
+        torch.cuda.synchronize()
+        out.append(a.elapsed_time(b))
+    return median(out)
+
+
+def tune_batch(batch, reps: int = 3, use_cache: bool = True) -> dict:
+    """Configure `batch` with the best known or best measured shape."""
+    import torch
+    arch = torch.cuda.get_device_properties(0).gcnArchName.split(":")[0]
+    key = f"{arch}|{batch.field.NAME}|{shape_bucket(batch)}"
+    db = _load() if use_cache else {}
+    if key in db:
+        cfg = db[key]["config"]
+        batch.configure(*cfg)
+        return {"config": cfg, "cached": True, "key": key}
+    rows, dws = candidates(batch.field.BYTES)
+    trials = {}
+    # Stage 1: tile shape at a middle launch shape.
+    for d in dws:
+        for r in rows:
+            cfg = (d, r, 4, 16, 1)
+            batch.configure(*cfg)
+            trials[cfg] = _time(batch, reps)
+    d, r = min(trials, key=trials.get)[:2]
+    # Stage 2: item size and residency for the winning tile.
+    for sub in (1, 4, 16):
+        for w in (8, 16, 32):
+            cfg = (d, r, sub, w, 1)
+            if cfg not in trials:
+                batch.configure(*cfg)
+                trials[cfg] = _time(batch, reps)
+    best = min(trials, key=trials.get)
+    batch.configure(*best)
+    if use_cache:
+        db[key] = {"config": list(best), "ms": round(trials[best], 4)}
+        _save(db)
+    return {"config": list(best), "cached": False, "key": key, "trials": len(trials),
+            "best_ms": round(trials[best], 4), "worst_ms": round(max(trials.values()), 4)}

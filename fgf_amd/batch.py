@@ -48,8 +48,12 @@ class Batch:
         _check(_lib.lib().fgf_batch_create(field.id, ctypes.addressof(arr), n, ctypes.byref(h)))
         self._h = h
 
-    def configure(self, dwords: int = 8, waves_per_cu: int = 16, order: int = 1) -> None:
-        _check(_lib.lib().fgf_batch_configure(self._h, dwords, waves_per_cu, order))
+    def configure(self, dwords: int = 4, rows_per_item: int = 8, subtiles: int = 4,
+                  waves_per_cu: int = 16, order: int = 1) -> None:
+        """Tile/launch shape: see fgf_batch_configure in include/fgf_amd.h."""
+        _check(_lib.lib().fgf_batch_configure(self._h, dwords, rows_per_item, subtiles,
+                                              waves_per_cu, order))
+        self.config = (dwords, rows_per_item, subtiles, waves_per_cu, order)
 
     def _s(self, stream):
         if stream is not None:
@@ -62,6 +66,10 @@ class Batch:
 
     def run_graph(self, stream=None) -> None:
         _check(_lib.lib().fgf_batch_run_graph(self._h, self._s(stream)))
+
+    def run_per_call(self, stream=None) -> None:
+        """Every stripe as its own per-call launch (the unbatched baseline)."""
+        _check(_lib.lib().fgf_batch_run_per_call(self._h, self._s(stream)))
 
     @property
     def items(self) -> int:

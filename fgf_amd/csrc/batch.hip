@@ -30,36 +30,28 @@ namespace {
 
 constexpr int BATCH_BLOCK = 256;
 
-// Runs the tile with the compile-time row count equal to `rows` (1..M).
-template <int B, int D, int... Ms>
-__device__ __forceinline__ void tile_rows(uint32_t rows, const StripeDesc &s, uint32_t j0,
-                                          uint64_t tile, uint32_t lane,
-                                          std::integer_sequence<int, Ms...>) {
-    ((rows == (uint32_t)(Ms + 1)
-          ? (core::matrix_tile<B, Ms + 1, D>(s.srcs, s.rows + j0,
-                                             s.tab + (size_t)j0 * s.tab_stride_j, s.tab_stride_t,
-                                             s.tab_stride_j, s.k, s.overwrite != 0, 0, s.row_len,
-                                             tile, lane),
-             0)
-          : 0),
-     ...);
-}
-
 template <int B, int M, int D>
 __global__ __launch_bounds__(BATCH_BLOCK) void k_batch(const BatchArgs a) {
     const uint32_t lane = threadIdx.x & 63u;
     constexpr uint64_t CHUNK = 256u * D;
-    for (;;) {
-        uint32_t it = 0;
-        if (lane == 0) it = atomicAdd(&a.counter[0], 1u);
-        it = __builtin_amdgcn_readfirstlane(it);
-        if (it >= a.nitems) break;
+    // The next item is claimed before the current one is processed: the
+    // atomic's latency overlaps with the current item's loads.
+    uint32_t claim = 0;
+    if (lane == 0) claim = atomicAdd(&a.counter[0], 1u);
+    uint32_t it = __builtin_amdgcn_readfirstlane(claim);
+    while (it < a.nitems) {
+        if (lane == 0) claim = atomicAdd(&a.counter[0], 1u);
         const WorkItem w = a.items[it];
         const StripeDesc s = a.stripes[w.stripe];
         const uint32_t j0 = w.rowgroup * M;
         const uint32_t mrows = min((uint32_t)M, s.m - j0);
-        tile_rows<B, D>(mrows, s, j0, (uint64_t)w.chunk * CHUNK, lane,
-                        std::make_integer_sequence<int, M>{});
+        uint64_t tile = (uint64_t)w.chunk * a.subtiles * CHUNK;
+        for (uint32_t c = 0; c < a.subtiles && tile < s.row_len; ++c, tile += CHUNK)
+            core::matrix_tile<B, M, D, true>(s.srcs, s.rows + j0,
+                                             s.tab + (size_t)j0 * s.tab_stride_j, s.tab_stride_t,
+                                             s.tab_stride_j, s.k, s.overwrite != 0, 0, s.row_len,
+                                             tile, lane, mrows);
+        it = __builtin_amdgcn_readfirstlane(claim);
     }
     // Retire: the last wave out resets the queue for the next launch.
     __threadfence();
@@ -74,10 +66,14 @@ __global__ __launch_bounds__(BATCH_BLOCK) void k_batch(const BatchArgs a) {
 
 using BatchFn = void (*)(BatchArgs);
 
-BatchFn pick_batch(int nb, int d) {
+BatchFn pick_batch(int nb, int d, int rows) {
     switch (nb) {
-    case 1: return d == 4 ? (BatchFn)k_batch<1, 8, 4> : (BatchFn)k_batch<1, 8, 8>;
-    case 2: return d == 4 ? (BatchFn)k_batch<2, 8, 4> : (BatchFn)k_batch<2, 8, 8>;
+    case 1:
+        if (rows == 4) return d == 4 ? (BatchFn)k_batch<1, 4, 4> : (BatchFn)k_batch<1, 4, 8>;
+        return d == 4 ? (BatchFn)k_batch<1, 8, 4> : (BatchFn)k_batch<1, 8, 8>;
+    case 2:
+        if (rows == 4) return d == 4 ? (BatchFn)k_batch<2, 4, 4> : (BatchFn)k_batch<2, 4, 8>;
+        return d == 4 ? (BatchFn)k_batch<2, 8, 4> : (BatchFn)k_batch<2, 8, 8>;
     case 4: return (BatchFn)k_batch<4, 4, 8>;
     default: return (BatchFn)k_batch<8, 2, 8>;
     }
@@ -85,26 +81,32 @@ BatchFn pick_batch(int nb, int d) {
 
 }  // namespace
 
-int batch_rows_per_item(int nb) { return nb <= 2 ? 8 : nb == 4 ? 4 : 2; }
+bool batch_rows_allowed(int nb, int rows) {
+    return nb <= 2 ? (rows == 4 || rows == 8) : rows == (nb == 4 ? 4 : 2);
+}
+
+int batch_default_rows(int nb) { return nb <= 2 ? 8 : nb == 4 ? 4 : 2; }
 
 int batch_min_dwords(int nb) { return nb >= 4 ? 8 : 4; }
 
-hipError_t launch_batch(int nb, int dwords, const BatchArgs &args, uint32_t blocks,
+hipError_t launch_batch(int nb, int dwords, int rows, const BatchArgs &args, uint32_t blocks,
                         hipStream_t stream) {
-    hipLaunchKernelGGL(pick_batch(nb, dwords), dim3(blocks), dim3(BATCH_BLOCK), 0, stream, args);
+    hipLaunchKernelGGL(pick_batch(nb, dwords, rows), dim3(blocks), dim3(BATCH_BLOCK), 0, stream,
+                       args);
     return hipGetLastError();
 }
 
 uint32_t batch_block_threads() { return BATCH_BLOCK; }
 
 // Host: the work list for a set of stripes. Items are (stripe, chunk, row
-// group); `order` 0 keeps stripe order, 1 sorts by descending per-item cost
+// group) where a chunk is `subtiles` wave tiles of 256 * dwords bytes;
+// `order` 0 keeps stripe order, 1 sorts by descending per-item cost
 // (longest-processing-time first) so the big items start early and the queue
 // drains evenly.
-std::vector<WorkItem> build_work_items(const std::vector<StripeDesc> &stripes, int nb, int dwords,
-                                       int order) {
-    const uint64_t chunk = 256u * (uint64_t)dwords;
-    const uint32_t rows = (uint32_t)batch_rows_per_item(nb);
+std::vector<WorkItem> build_work_items(const std::vector<StripeDesc> &stripes, int dwords,
+                                       int rows_per_item, int subtiles, int order) {
+    const uint64_t chunk = 256u * (uint64_t)dwords * (uint64_t)subtiles;
+    const uint32_t rows = (uint32_t)rows_per_item;
     std::vector<WorkItem> items;
     for (uint32_t s = 0; s < stripes.size(); ++s) {
         const StripeDesc &d = stripes[s];
@@ -116,7 +118,8 @@ std::vector<WorkItem> build_work_items(const std::vector<StripeDesc> &stripes, i
     if (order == 1) {
         auto cost = [&](const WorkItem &w) {
             const StripeDesc &d = stripes[w.stripe];
-            return (uint64_t)d.k * std::min<uint32_t>(rows, d.m - w.rowgroup * rows);
+            const uint64_t cols = std::min<uint64_t>(chunk, d.row_len - (uint64_t)w.chunk * chunk);
+            return cols * d.k * std::min<uint32_t>(rows, d.m - w.rowgroup * rows);
         };
         std::stable_sort(items.begin(), items.end(),
                          [&](const WorkItem &x, const WorkItem &y) { return cost(x) > cost(y); });
@@ -137,6 +140,8 @@ struct fgf_batch {
     int nb = 1;
     int dwords = 8;
     int order = 1;
+    int rows_per_item = 8;
+    int subtiles = 4;
     uint32_t waves_per_cu = 16;
     size_t nstripes = 0;
     std::vector<fgf::StripeDesc> stripes;   // host copy (device pointers inside)
@@ -144,6 +149,8 @@ struct fgf_batch {
     fgf::WorkItem *d_items = nullptr;
     size_t nitems = 0, items_cap = 0;
     void *d_ptrs = nullptr;                  // all srcs/rows pointer arrays
+    std::vector<const void *> host_ptrs;     // the same arrays on the host
+    std::vector<size_t> host_offs;           // stripe s: srcs at host_offs[s], rows after
     uint32_t *d_counter = nullptr;
     uint32_t blocks = 0;
     uint64_t bytes = 0;                      // algorithmic bytes per run
@@ -178,7 +185,8 @@ void batch_release_graph(fgf_batch *b) {
 }
 
 int batch_build_items(fgf_batch *b) {
-    std::vector<fgf::WorkItem> items = fgf::build_work_items(b->stripes, b->nb, b->dwords, b->order);
+    std::vector<fgf::WorkItem> items =
+        fgf::build_work_items(b->stripes, b->dwords, b->rows_per_item, b->subtiles, b->order);
     b->nitems = items.size();
     if (b->nitems > b->items_cap) {
         if (b->d_items) (void)hipFree(b->d_items);
@@ -242,7 +250,8 @@ int fgf_batch_create(fgf_field f, const fgf_stripe *stripes, size_t n, fgf_batch
     if (!b) return bfail("out of host memory");
     b->field = f;
     b->nb = nb;
-    b->dwords = 8;
+    b->dwords = fgf::batch_min_dwords(nb) == 4 ? 4 : 8;
+    b->rows_per_item = fgf::batch_default_rows(nb);
     b->nstripes = n;
     std::vector<const void *> ptrs;
     ptrs.reserve(nptrs);
@@ -261,6 +270,8 @@ int fgf_batch_create(fgf_field f, const fgf_stripe *stripes, size_t n, fgf_batch
     }
     const void **dp = (const void **)b->d_ptrs;
     b->stripes.resize(n);
+    b->host_ptrs = ptrs;
+    b->host_offs = offs;
     const size_t e_dw = (size_t)nb * nb * TAB_DW;
     for (size_t s = 0; s < n; ++s) {
         const fgf_stripe &st = stripes[s];
@@ -293,14 +304,21 @@ int fgf_batch_create(fgf_field f, const fgf_stripe *stripes, size_t n, fgf_batch
     return FGF_OK;
 }
 
-int fgf_batch_configure(fgf_batch *b, int dwords, int waves_per_cu, int order) {
+int fgf_batch_configure(fgf_batch *b, int dwords, int rows_per_item, int subtiles,
+                        int waves_per_cu, int order) {
     if (!b) return bfail("fgf_batch_configure: batch is null");
     if (dwords != 4 && dwords != 8) return bfail("fgf_batch_configure: dwords must be 4 or 8");
+    if (!fgf::batch_rows_allowed(b->nb, rows_per_item))
+        return bfail("fgf_batch_configure: %d rows per item is not available for %d-byte elements",
+                     rows_per_item, b->nb);
+    if (subtiles < 1 || subtiles > 64) return bfail("fgf_batch_configure: subtiles must be in 1..64");
     if (dwords < fgf::batch_min_dwords(b->nb))
         return bfail("fgf_batch_configure: %d-byte elements need dwords >= %d", b->nb, fgf::batch_min_dwords(b->nb));
     if (waves_per_cu < 1 || waves_per_cu > 32) return bfail("fgf_batch_configure: waves_per_cu must be in 1..32");
     if (order != 0 && order != 1) return bfail("fgf_batch_configure: order must be 0 or 1");
     b->dwords = dwords;
+    b->rows_per_item = rows_per_item;
+    b->subtiles = subtiles;
     b->waves_per_cu = (uint32_t)waves_per_cu;
     b->order = order;
     return batch_build_items(b);
@@ -310,8 +328,10 @@ int fgf_batch_run(fgf_batch *b, void *stream) {
     if (!b) return bfail("fgf_batch_run: batch is null");
     if (b->nitems == 0) return FGF_OK;
     fgf::BatchArgs a{b->d_stripes, b->d_items, (uint32_t)b->nitems,
-                     b->blocks * (fgf::batch_block_threads() / 64), b->d_counter};
-    hipError_t e = fgf::launch_batch(b->nb, b->dwords, a, b->blocks, (hipStream_t)stream);
+                     b->blocks * (fgf::batch_block_threads() / 64), (uint32_t)b->subtiles,
+                     b->d_counter};
+    hipError_t e = fgf::launch_batch(b->nb, b->dwords, b->rows_per_item, a, b->blocks,
+                                     (hipStream_t)stream);
     return e == hipSuccess ? FGF_OK : bdev("batch launch", e);
 }
 
@@ -344,6 +364,29 @@ int fgf_batch_run_graph(fgf_batch *b, void *stream) {
     }
     hipError_t e = hipGraphLaunch(b->exec, s);
     return e == hipSuccess ? FGF_OK : bdev("hipGraphLaunch", e);
+}
+
+int fgf_batch_run_per_call(fgf_batch *b, void *stream) {
+    if (!b) return bfail("fgf_batch_run_per_call: batch is null");
+    for (size_t s = 0; s < b->nstripes; ++s) {
+        const fgf::StripeDesc &d = b->stripes[s];
+        const void *const *p = b->host_ptrs.data() + b->host_offs[s];
+        fgf::MatrixJob job;
+        job.nb = b->nb;
+        job.srcs = (const uint8_t *const *)p;
+        job.k = d.k;
+        job.rows = (uint8_t *const *)(p + d.k);
+        job.m = d.m;
+        job.tab = d.tab;
+        job.tab_stride_t = d.tab_stride_t;
+        job.tab_stride_j = d.tab_stride_j;
+        job.row_len = d.row_len;
+        job.overwrite = d.overwrite != 0;
+        job.stream = (hipStream_t)stream;
+        hipError_t e = fgf::launch_matrix(job);
+        if (e != hipSuccess) return bdev("per-call launch", e);
+    }
+    return FGF_OK;
 }
 
 size_t fgf_batch_items(const fgf_batch *b) { return b ? b->nitems : 0; }

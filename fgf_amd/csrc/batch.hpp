@@ -22,7 +22,7 @@ struct StripeDesc {
 
 struct WorkItem {
     uint32_t stripe;
-    uint32_t chunk;      // column chunk of 256 * dwords bytes
+    uint32_t chunk;      // column chunk of subtiles * 256 * dwords bytes
     uint32_t rowgroup;   // rows [rowgroup * M, rowgroup * M + M)
     uint32_t pad;
 };
@@ -32,15 +32,17 @@ struct BatchArgs {
     const WorkItem *items;
     uint32_t nitems;
     uint32_t total_waves;
+    uint32_t subtiles;   // wave tiles per item
     uint32_t *counter;   // [0] next item, [1] retired waves; zero between launches
 };
 
-int batch_rows_per_item(int nb);
+bool batch_rows_allowed(int nb, int rows);
+int batch_default_rows(int nb);
 int batch_min_dwords(int nb);
 uint32_t batch_block_threads();
-hipError_t launch_batch(int nb, int dwords, const BatchArgs &args, uint32_t blocks,
+hipError_t launch_batch(int nb, int dwords, int rows, const BatchArgs &args, uint32_t blocks,
                         hipStream_t stream);
-std::vector<WorkItem> build_work_items(const std::vector<StripeDesc> &stripes, int nb, int dwords,
-                                       int order);
+std::vector<WorkItem> build_work_items(const std::vector<StripeDesc> &stripes, int dwords,
+                                       int rows_per_item, int subtiles, int order);
 
 }  // namespace fgf

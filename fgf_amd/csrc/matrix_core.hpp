@@ -168,16 +168,18 @@ __device__ __forceinline__ void load_source(uint32_t (&x)[D], SrcPtrs src, uint3
 //  - dword-outer: one dword's selectors live, every table of the step held
 //    in SGPRs (S*M*5 dwords) -- byte fields with few rows;
 //  - row-outer: all selectors of the step live, one row's tables at a time.
+// With GUARD, only rows j < mrows are folded (row-outer, so each row's tables
+// are still loaded once): one body serves every row count up to M.
 template <int B, int M>
 constexpr bool dword_outer() { return B == 1 && M <= 6; }
 
-template <int B, int M, int D, int S>
+template <int B, int M, int D, int S, bool GUARD>
 __device__ __forceinline__ void consume(uint32_t (&acc)[M][D], const uint32_t *x0,
                                         const uint32_t *x1, const uint32_t *__restrict__ tt,
-                                        uint32_t stride_t, uint32_t stride_j) {
+                                        uint32_t stride_t, uint32_t stride_j, uint32_t mrows) {
     constexpr int G = D / B;
     const uint32_t *xs[2] = {x0, x1};
-    if constexpr (dword_outer<B, M>()) {
+    if constexpr (!GUARD && dword_outer<B, M>()) {
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             Sel sel[S][B];
@@ -216,6 +218,7 @@ __device__ __forceinline__ void consume(uint32_t (&acc)[M][D], const uint32_t *x
             }
 #pragma unroll
         for (int j = 0; j < M; ++j) {
+            if (GUARD && j >= (int)mrows) continue;
 #pragma unroll
             for (int p = 0; p < B; ++p) {
 #pragma unroll
@@ -235,37 +238,39 @@ __device__ __forceinline__ void consume(uint32_t (&acc)[M][D], const uint32_t *x
 }
 
 // Folds the sources t (and t + 1 when present) held in x.
-template <int B, int M, int D>
+template <int B, int M, int D, bool GUARD>
 __device__ __forceinline__ void consume_step(uint32_t (&acc)[M][D], const uint32_t (&x)[2][D],
                                              const uint32_t *__restrict__ tab, uint32_t t,
-                                             uint32_t k, uint32_t stride_t, uint32_t stride_j) {
+                                             uint32_t k, uint32_t stride_t, uint32_t stride_j,
+                                             uint32_t mrows) {
     const uint32_t *tt = tab + __builtin_amdgcn_readfirstlane(t * stride_t);
     if (t + 1 < k) {
         if constexpr (B == 1) {
-            consume<B, M, D, 2>(acc, x[0], x[1], tt, stride_t, stride_j);
+            consume<B, M, D, 2, GUARD>(acc, x[0], x[1], tt, stride_t, stride_j, mrows);
         } else {
-            consume<B, M, D, 1>(acc, x[0], x[0], tt, stride_t, stride_j);
-            consume<B, M, D, 1>(acc, x[1], x[1], tt + stride_t, stride_t, stride_j);
+            consume<B, M, D, 1, GUARD>(acc, x[0], x[0], tt, stride_t, stride_j, mrows);
+            consume<B, M, D, 1, GUARD>(acc, x[1], x[1], tt + stride_t, stride_t, stride_j, mrows);
         }
     } else {
-        consume<B, M, D, 1>(acc, x[0], x[0], tt, stride_t, stride_j);
+        consume<B, M, D, 1, GUARD>(acc, x[0], x[0], tt, stride_t, stride_j, mrows);
     }
 }
 
 // One wave's tile of rows[j] (=|^=) sum_t c(t, j) * src[t], for columns
 // [col0 + tile, col0 + tile + 256*D) clipped to col0 + ncols. All pointers
-// + col0 must be 16-byte aligned and ncols a multiple of 16. Exactly M rows
-// are processed (a compile-time count keeps every table load hoistable out
-// of the dword loop); `ow` selects overwrite (zero seed) or accumulate
+// + col0 must be 16-byte aligned and ncols a multiple of 16. Without GUARD
+// exactly M rows are processed (a compile-time count keeps every table load
+// hoistable out of the dword loop); with GUARD rows j < mrows (<= M) are.
+// `ow` selects overwrite (zero seed) or accumulate
 // (seed from the destination). Sources are consumed in pairs from two
 // ping-pong register buffers: the next pair's loads are in flight while the
 // current pair is multiplied.
-template <int B, int M, int D, typename SrcPtrs, typename DstPtrs>
+template <int B, int M, int D, bool GUARD = false, typename SrcPtrs, typename DstPtrs>
 __device__ __forceinline__ void matrix_tile(SrcPtrs src, DstPtrs dst,
                                             const uint32_t *__restrict__ tab, uint32_t stride_t,
                                             uint32_t stride_j, uint32_t k, bool ow,
                                             uint64_t col0, uint64_t ncols, uint64_t tile,
-                                            uint32_t lane) {
+                                            uint32_t lane, uint32_t mrows = M) {
     constexpr int P = D / 4;
     constexpr int G = D / B;
     static_assert(D % 4 == 0 && D % B == 0, "tile shape");
@@ -285,7 +290,8 @@ __device__ __forceinline__ void matrix_tile(SrcPtrs src, DstPtrs dst,
 #pragma unroll
         for (int i = 0; i < P; ++i) {
             w[4 * i + 0] = w[4 * i + 1] = w[4 * i + 2] = w[4 * i + 3] = 0;
-            if (!ow && tl.ok[i]) load16(&w[4 * i], dst[j] + base + tl.lo[i]);
+            if (!ow && tl.ok[i] && (!GUARD || j < (int)mrows))
+                load16(&w[4 * i], dst[j] + base + tl.lo[i]);
         }
 #pragma unroll
         for (int g = 0; g < G; ++g) to_planes<B>(&w[g * B], &acc[j][g * B]);
@@ -297,18 +303,19 @@ __device__ __forceinline__ void matrix_tile(SrcPtrs src, DstPtrs dst,
     for (uint32_t t = 0;;) {
         if (t + 2 < k) load_source<D>(xb[0], src, t + 2, base, tl);
         if (t + 3 < k) load_source<D>(xb[1], src, t + 3, base, tl);
-        consume_step<B, M, D>(acc, xa, tab, t, k, stride_t, stride_j);
+        consume_step<B, M, D, GUARD>(acc, xa, tab, t, k, stride_t, stride_j, mrows);
         t += 2;
         if (t >= k) break;
         if (t + 2 < k) load_source<D>(xa[0], src, t + 2, base, tl);
         if (t + 3 < k) load_source<D>(xa[1], src, t + 3, base, tl);
-        consume_step<B, M, D>(acc, xb, tab, t, k, stride_t, stride_j);
+        consume_step<B, M, D, GUARD>(acc, xb, tab, t, k, stride_t, stride_j, mrows);
         t += 2;
         if (t >= k) break;
     }
 
 #pragma unroll
     for (int j = 0; j < M; ++j) {
+        if (GUARD && j >= (int)mrows) break;
         uint32_t w[D];
 #pragma unroll
         for (int g = 0; g < G; ++g) from_planes<B>(&acc[j][g * B], &w[g * B]);

@@ -220,13 +220,21 @@ typedef struct fgf_stripe {
 /* Validate the stripes and upload the descriptor queue to the current device. */
 int fgf_batch_create(fgf_field field, const fgf_stripe *stripes, size_t nstripes,
                      fgf_batch **out);
-/* Tile shape and launch shape: dwords per lane per source (4 or 8), resident
- * waves per CU (1..32), work order (0 = stripe order, 1 = largest items first). */
-int fgf_batch_configure(fgf_batch *batch, int dwords, int waves_per_cu, int order);
+/* Tile and launch shape (the autotuner's knobs): dwords per lane per source
+ * (4 or 8; 8 for 4- and 8-byte elements), destination rows per wave (4 or 8
+ * for 1- and 2-byte elements, 4 for GF(2^32), 2 for GF(2^64)), wave tiles per
+ * work item (1..64), resident waves per CU (1..32), and work order
+ * (0 = stripe order, 1 = largest items first). */
+int fgf_batch_configure(fgf_batch *batch, int dwords, int rows_per_item, int subtiles,
+                        int waves_per_cu, int order);
 /* One persistent launch over every stripe, on `stream`. */
 int fgf_batch_run(fgf_batch *batch, void *stream);
 /* The same launch through a hipGraph captured on first use and replayed after. */
 int fgf_batch_run_graph(fgf_batch *batch, void *stream);
+/* The same stripes as one per-call launch sequence (fgf_dot_product_matrix_with /
+ * fgf_mul_add_matrix_with semantics per stripe), issued from a host loop: the
+ * unbatched comparison point for the persistent launch. */
+int fgf_batch_run_per_call(fgf_batch *batch, void *stream);
 size_t fgf_batch_items(const fgf_batch *batch);
 /* Algorithmic HBM bytes of one run: sum of (k + m) * L (overwrite) or (k + 2m) * L. */
 uint64_t fgf_batch_bytes(const fgf_batch *batch);

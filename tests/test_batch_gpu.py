@@ -46,9 +46,9 @@ def test_batch_matches_per_call_and_restores_erasures(field):
     pool, saved = workloads.materialize(field, specs, seed=12)
     batch, plans = build(field, specs, pool)
     want = [out.clone() for _, out in per_call(field, specs, pool, plans)]
-    for cfg in [(8, 16, 1), (4, 8, 0), (8, 4, 0), (4, 32, 1)]:
-        if field.BYTES == 1 or cfg[0] >= 4:
-            batch.configure(*cfg)
+    # (dwords, rows per item, subtiles, waves per CU, order)
+    for cfg in [(8, 8, 1, 16, 1), (4, 4, 4, 8, 0), (8, 4, 16, 4, 0), (4, 8, 2, 32, 1)]:
+        batch.configure(*cfg)
         for runner in (batch.run, batch.run_graph, batch.run_graph):
             for i, sp in enumerate(specs):               # clobber the outputs first
                 for r in workloads.stripe_buffers(pool, sp)[1]:
