@@ -165,6 +165,75 @@ def bench_batched(n_stripes: int = 4096, seed: int = 4096, reps: int = 5) -> dic
     }
 
 
+def bench_serving(n_requests: int = 4096, seed: int = 77,
+                  rates=(2000.0, 10000.0, 50000.0, 200000.0, None)) -> dict:
+    """Open-loop serving: Poisson arrivals at each offered rate (None = all at
+    once) through the continuous-batching server.
+
+    Requests: 80% Gf8D / 20% Gf16 over 256 distinct coefficient matrices
+    (k ~ U[4, 32], m ~ U[2, 8], half of the matrices reconstruct a random set
+    of erased shards), shard length log-uniform on [4 KiB, 1 MiB]. Latency runs
+    from a request's scheduled arrival to the host seeing its launch finish.
+    Every leg's outputs are checked byte for byte against the per-call results.
+    """
+    import numpy as np
+    import torch
+
+    from fgf_amd import Gf8D, Gf16, workloads
+    from fgf_amd.serve import Server
+
+    specs = workloads.draw_serving_mix([(Gf8D, 0.8), (Gf16, 0.2)], n_requests, seed)
+    pool, expected = workloads.materialize_serving(specs, seed + 1)
+    reqs = [workloads.serve_request(pool, sp) for sp in specs]
+    total = sum((sp.k + len(sp.out_shards)) * sp.row_len for sp in specs)
+    srv = Server()
+    srv.replay(reqs[:256], [0.0] * 256)             # warm: plans, slot buffers, code objects
+    legs = []
+    rng = np.random.default_rng(seed + 2)
+    for rate in rates:
+        for sp in specs:
+            for i in sp.out_shards:
+                workloads.serve_shard(pool, sp, i).zero_()
+        torch.cuda.synchronize()
+        if rate is None:
+            arrivals = [0.0] * n_requests
+        else:
+            arrivals = np.cumsum(rng.exponential(1.0 / rate, n_requests)).tolist()
+        before = srv.stats()
+        lat, wall = srv.replay(reqs, arrivals)
+        after = srv.stats()
+        ok = all(torch.equal(torch.cat([workloads.serve_shard(pool, sp, i) for i in sp.out_shards]), w)
+                 for sp, w in zip(specs, expected))
+        assert ok, f"served output differs from the per-call result at rate {rate}"
+        lat_ms = np.array(lat) * 1e3
+        launches = after["launches"] - before["launches"]
+        legs.append({
+            "offered_req_per_s": rate if rate is not None else "burst",
+            "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+            "p999_ms": round(float(np.percentile(lat_ms, 99.9)), 3),
+            "GBps": round(total / wall / 1e9, 1),
+            "wall_s": round(wall, 3),
+            "launches": launches,
+            "stripes_per_launch": round(n_requests / max(launches, 1), 1),
+            "admission_waits": after["admission_waits"] - before["admission_waits"],
+        })
+    st = srv.stats()
+    srv.close()
+    del pool
+    torch.cuda.empty_cache()
+    return {
+        "workload": f"{n_requests} requests, 80% Gf8D / 20% Gf16, 256 coefficient matrices, "
+                    f"k~U[4,32], m~U[2,8], L log-uniform 4KiB-1MiB, 50% reconstruct matrices",
+        "algorithmic_bytes": total,
+        "server": {"slots": 4, "hbm_budget_GiB": 64, "max_batch_stripes": 256},
+        "legs": legs,
+        "plan_cache": {"hits": st["plan_hits"], "misses": st["plan_misses"],
+                       "evictions": st["plan_evictions"]},
+        "bit_exact_vs_per_call": True,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,6 +241,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-mix leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--row-len", type=int, default=L_HEADLINE)
     ap.add_argument("--calib-copy", action="store_true",
                     help="after timing, run one 1 GiB device copy (a plain copy kernel of known "
@@ -272,6 +342,10 @@ def main() -> int:
     if not args.no_batched:
         print("[bench] batched uneven-stripe mix", file=sys.stderr, flush=True)
         out["batched"] = bench_batched()
+    if not args.no_serving:
+        print("[bench] serving: Poisson arrivals through the batching server", file=sys.stderr,
+              flush=True)
+        out["serving"] = bench_serving()
     if not args.no_cpu:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline()

@@ -72,6 +72,14 @@ PROTOTYPES = {
     "fgf_batch_run_graph": (c_int, [c_vp, c_vp]),
     "fgf_batch_run_per_call": (c_int, [c_vp, c_vp]),
     "fgf_batch_items": (c_size, [c_vp]),
+    "fgf_server_create": (c_int, [c_vp, ctypes.POINTER(c_vp)]),
+    "fgf_server_submit": (c_int, [c_vp, c_vp, ctypes.POINTER(c_u64)]),
+    "fgf_server_drain": (c_int, [c_vp]),
+    "fgf_server_result": (c_int, [c_vp, c_u64, ctypes.POINTER(ctypes.c_double),
+                                  ctypes.POINTER(c_int)]),
+    "fgf_server_stats_get": (c_int, [c_vp, c_vp]),
+    "fgf_server_replay": (c_int, [c_vp, c_vp, c_vp, c_size, c_vp, ctypes.POINTER(ctypes.c_double)]),
+    "fgf_server_destroy": (None, [c_vp]),
     "fgf_batch_bytes": (ctypes.c_uint64, [c_vp]),
     "fgf_batch_free": (None, [c_vp]),
     "fgf_pack": (c_int, [c_int, c_vp, c_size, ctypes.POINTER(c_u64), c_size]),

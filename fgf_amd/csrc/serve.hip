@@ -1,0 +1,586 @@
+// Erasure-coding service over the batched kernel: continuous batching with
+// HBM-budget admission, per-slot streams and a device plan cache.
+//
+// Requests queue in arrival order. A scheduler thread loops:
+//   1. retire slots whose launch has finished (hipEventQuery): stamp the
+//      completion time of their requests, release their budget and plans;
+//   2. if a slot is free, admit queued requests of the head request's field
+//      in order while the admitted-but-unfinished bytes stay within the HBM
+//      budget and the slot has room, resolve their coefficient plans through
+//      the cache, upload the slot's descriptors from pinned staging and
+//      launch the persistent kernel on the slot's stream.
+// Launches on different slots overlap on the device, so requests that arrive
+// while others run start as soon as a slot and budget are free.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <list>
+#include <mutex>
+#include <new>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "batch.hpp"
+#include "fgf_amd.h"
+#include "field.hpp"
+#include "kernels.hpp"
+#include "plan.hpp"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+int sfail(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+int sfail(const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    fgf::set_last_error(buf);
+    return FGF_EINVAL;
+}
+
+int sdev(const char *what, hipError_t e) {
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    fgf::set_last_error(m.c_str());
+    return FGF_EDEVICE;
+}
+
+struct Req {
+    uint64_t ticket;
+    fgf_request r;
+    std::vector<uint8_t *> shards;
+    std::vector<uint32_t> erased;
+    uint64_t bytes;
+    Clock::time_point submitted;
+};
+
+struct PlanEntry {
+    fgf_plan *plan = nullptr;
+    uint32_t inflight = 0;
+    std::list<std::string>::iterator lru;
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    bool busy = false;
+    size_t cap_stripes = 0, cap_items = 0, cap_ptrs = 0;
+    fgf::StripeDesc *d_stripes = nullptr, *h_stripes = nullptr;
+    fgf::WorkItem *d_items = nullptr, *h_items = nullptr;
+    const void **d_ptrs = nullptr, **h_ptrs = nullptr;
+    uint32_t *d_counter = nullptr;
+    std::vector<Req> reqs;
+    std::vector<PlanEntry *> plans;
+    uint64_t bytes = 0;
+};
+
+}  // namespace
+
+struct fgf_server {
+    fgf_server_config cfg{};
+    int device = 0;
+    int cus = 256;
+    std::vector<Slot> slots;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Req> queue;
+    bool stop = false;
+    uint64_t next_ticket = 0;
+    uint64_t inflight_bytes = 0;
+    std::vector<double> latency;   // per ticket, -1 while pending
+    std::vector<int> status;
+    fgf_server_stats stats{};
+    std::unordered_map<std::string, PlanEntry> cache;
+    std::list<std::string> lru;
+    std::thread worker;
+};
+
+namespace {
+
+using namespace fgf;
+
+// Host matrices for the systematic code [I_k ; C], C[i][j] = 1/(i ^ (m + j)).
+uint64_t cauchy(int f, uint32_t m, uint32_t i, uint32_t j) {
+    return HostField::inv(f, (uint64_t)(i ^ (m + j)));
+}
+
+std::vector<uint64_t> generator_row(int f, uint32_t k, uint32_t m, uint32_t shard) {
+    std::vector<uint64_t> row(k, 0);
+    if (shard < k)
+        row[shard] = 1;
+    else
+        for (uint32_t j = 0; j < k; ++j) row[j] = cauchy(f, m, shard - k, j);
+    return row;
+}
+
+// Gauss-Jordan inverse over the field; false if singular.
+bool invert(int f, std::vector<uint64_t> a, uint32_t n, std::vector<uint64_t> &inv) {
+    inv.assign((size_t)n * n, 0);
+    for (uint32_t i = 0; i < n; ++i) inv[(size_t)i * n + i] = 1;
+    for (uint32_t c = 0; c < n; ++c) {
+        uint32_t p = c;
+        while (p < n && a[(size_t)p * n + c] == 0) ++p;
+        if (p == n) return false;
+        if (p != c)
+            for (uint32_t x = 0; x < n; ++x) {
+                std::swap(a[(size_t)c * n + x], a[(size_t)p * n + x]);
+                std::swap(inv[(size_t)c * n + x], inv[(size_t)p * n + x]);
+            }
+        const uint64_t s = HostField::inv(f, a[(size_t)c * n + c]);
+        for (uint32_t x = 0; x < n; ++x) {
+            a[(size_t)c * n + x] = HostField::mul(f, a[(size_t)c * n + x], s);
+            inv[(size_t)c * n + x] = HostField::mul(f, inv[(size_t)c * n + x], s);
+        }
+        for (uint32_t r = 0; r < n; ++r) {
+            const uint64_t g = a[(size_t)r * n + c];
+            if (r == c || g == 0) continue;
+            for (uint32_t x = 0; x < n; ++x) {
+                a[(size_t)r * n + x] ^= HostField::mul(f, g, a[(size_t)c * n + x]);
+                inv[(size_t)r * n + x] ^= HostField::mul(f, g, inv[(size_t)c * n + x]);
+            }
+        }
+    }
+    return true;
+}
+
+// Survivors: the first k shards that are not erased.
+std::vector<uint32_t> survivors_of(const Req &q) {
+    std::vector<uint32_t> s;
+    for (uint32_t i = 0; i < q.r.k + q.r.m && s.size() < q.r.k; ++i)
+        if (std::find(q.erased.begin(), q.erased.end(), i) == q.erased.end()) s.push_back(i);
+    return s;
+}
+
+std::string plan_key(const Req &q) {
+    std::string k = std::to_string(q.r.field) + (q.r.reconstruct ? "R" : "E") + std::to_string(q.r.k) +
+                    "," + std::to_string(q.r.m);
+    for (uint32_t e : q.erased) k += ":" + std::to_string(e);
+    return k;
+}
+
+// Plan values (sources, rows) for a request: encode rows are the parity
+// shards; decode rows are G[e] * S^-1 over the k survivors S.
+bool plan_values(const Req &q, std::vector<uint64_t> &vals, uint32_t &rows) {
+    const int f = q.r.field;
+    const uint32_t k = q.r.k, m = q.r.m;
+    if (!q.r.reconstruct) {
+        rows = m;
+        vals.resize((size_t)k * m);
+        for (uint32_t t = 0; t < k; ++t)
+            for (uint32_t j = 0; j < m; ++j) vals[(size_t)t * m + j] = cauchy(f, m, j, t);
+        return true;
+    }
+    const std::vector<uint32_t> surv = survivors_of(q);
+    std::vector<uint64_t> s((size_t)k * k), sinv;
+    for (uint32_t r = 0; r < k; ++r) {
+        const std::vector<uint64_t> g = generator_row(f, k, m, surv[r]);
+        std::copy(g.begin(), g.end(), s.begin() + (size_t)r * k);
+    }
+    if (!invert(f, s, k, sinv)) return false;
+    rows = (uint32_t)q.erased.size();
+    vals.assign((size_t)k * rows, 0);
+    for (uint32_t j = 0; j < rows; ++j) {
+        const std::vector<uint64_t> g = generator_row(f, k, m, q.erased[j]);
+        for (uint32_t t = 0; t < k; ++t) {
+            uint64_t acc = 0;
+            for (uint32_t x = 0; x < k; ++x) acc ^= HostField::mul(f, g[x], sinv[(size_t)x * k + t]);
+            vals[(size_t)t * rows + j] = acc;
+        }
+    }
+    return true;
+}
+
+PlanEntry *acquire_plan(fgf_server *s, const Req &q) {
+    const std::string key = plan_key(q);
+    auto it = s->cache.find(key);
+    if (it != s->cache.end()) {
+        s->lru.splice(s->lru.begin(), s->lru, it->second.lru);
+        ++s->stats.plan_hits;
+        ++it->second.inflight;
+        return &it->second;
+    }
+    ++s->stats.plan_misses;
+    std::vector<uint64_t> vals;
+    uint32_t rows = 0;
+    if (!plan_values(q, vals, rows)) return nullptr;
+    const int nb = field_bytes(q.r.field);
+    std::vector<uint8_t> raw(vals.size() * nb);
+    for (size_t i = 0; i < vals.size(); ++i)
+        for (int b = 0; b < nb; ++b) raw[i * nb + b] = (uint8_t)(vals[i] >> (8 * b));
+    fgf_plan *p = nullptr;
+    if (fgf_plan_matrix((fgf_field)q.r.field, q.r.k, rows, raw.data(), vals.size(), &p) != FGF_OK)
+        return nullptr;
+    // Evict least-recently-used plans that no launch is reading.
+    while (s->cache.size() >= s->cfg.plan_cache_entries) {
+        auto victim = std::find_if(s->lru.rbegin(), s->lru.rend(),
+                                   [&](const std::string &k2) { return s->cache[k2].inflight == 0; });
+        if (victim == s->lru.rend()) break;
+        auto ce = s->cache.find(*victim);
+        fgf_plan_free(ce->second.plan);
+        s->lru.erase(std::next(victim).base());
+        s->cache.erase(ce);
+        ++s->stats.plan_evictions;
+    }
+    s->lru.push_front(key);
+    PlanEntry &e = s->cache[key];
+    e.plan = p;
+    e.inflight = 1;
+    e.lru = s->lru.begin();
+    return &e;
+}
+
+template <typename T>
+hipError_t grow(T *&dev, T *&host, size_t &cap, size_t need) {
+    if (need <= cap) return hipSuccess;
+    size_t n = std::max<size_t>(need, cap * 2);
+    if (dev) (void)hipFree(dev);
+    if (host) (void)hipHostFree(host);
+    dev = nullptr;
+    host = nullptr;
+    hipError_t e = hipMalloc((void **)&dev, n * sizeof(T));
+    if (e == hipSuccess) e = hipHostMalloc((void **)&host, n * sizeof(T), hipHostMallocDefault);
+    if (e == hipSuccess) cap = n;
+    return e;
+}
+
+void finish_slot(fgf_server *s, Slot &sl, int status) {
+    const Clock::time_point now = Clock::now();
+    for (Req &q : sl.reqs) {
+        s->latency[q.ticket] = std::chrono::duration<double>(now - q.submitted).count();
+        s->status[q.ticket] = status;
+        if (status == 0) {
+            ++s->stats.completed;
+            s->stats.bytes += q.bytes;
+        } else {
+            ++s->stats.failed;
+        }
+    }
+    for (PlanEntry *p : sl.plans) --p->inflight;
+    s->inflight_bytes -= sl.bytes;
+    sl.reqs.clear();
+    sl.plans.clear();
+    sl.bytes = 0;
+    sl.busy = false;
+}
+
+// Builds and launches one slot's batch from the admitted requests.
+int launch_slot(fgf_server *s, Slot &sl) {
+    const int f = sl.reqs.front().r.field;
+    const int nb = field_bytes(f);
+    std::vector<StripeDesc> stripes(sl.reqs.size());
+    size_t nptrs = 0;
+    for (const Req &q : sl.reqs) nptrs += q.r.k + q.r.m;
+    hipError_t e = grow(sl.d_ptrs, sl.h_ptrs, sl.cap_ptrs, nptrs);
+    if (e == hipSuccess) e = grow(sl.d_stripes, sl.h_stripes, sl.cap_stripes, stripes.size());
+    if (e != hipSuccess) return sdev("slot buffers", e);
+    size_t p = 0;
+    const size_t edw = (size_t)nb * nb * TAB_DW;
+    for (size_t i = 0; i < sl.reqs.size(); ++i) {
+        const Req &q = sl.reqs[i];
+        const fgf_plan *plan = sl.plans[i]->plan;
+        StripeDesc &d = stripes[i];
+        const size_t p0 = p;
+        if (!q.r.reconstruct) {
+            for (uint32_t t = 0; t < q.r.k; ++t) sl.h_ptrs[p++] = q.shards[t];
+            for (uint32_t j = 0; j < q.r.m; ++j) sl.h_ptrs[p++] = q.shards[q.r.k + j];
+        } else {
+            for (uint32_t t : survivors_of(q)) sl.h_ptrs[p++] = q.shards[t];
+            for (uint32_t e2 : q.erased) sl.h_ptrs[p++] = q.shards[e2];
+        }
+        d.srcs = (const uint8_t *const *)(sl.d_ptrs + p0);
+        d.rows = (uint8_t *const *)(sl.d_ptrs + p0 + plan->rows);
+        d.tab = plan->d_tab;
+        d.row_len = q.r.row_len;
+        d.k = (uint32_t)plan->rows;
+        d.m = (uint32_t)plan->cols;
+        d.tab_stride_t = (uint32_t)(plan->cols * edw);
+        d.tab_stride_j = (uint32_t)edw;
+        d.overwrite = 1;
+        d.pad = 0;
+    }
+    std::vector<WorkItem> items =
+        build_work_items(stripes, s->cfg.dwords, s->cfg.rows_per_item, s->cfg.subtiles, 1);
+    if ((e = grow(sl.d_items, sl.h_items, sl.cap_items, items.size())) != hipSuccess)
+        return sdev("slot buffers", e);
+    std::memcpy(sl.h_stripes, stripes.data(), stripes.size() * sizeof(StripeDesc));
+    std::memcpy(sl.h_items, items.data(), items.size() * sizeof(WorkItem));
+    if ((e = hipMemcpyAsync(sl.d_ptrs, sl.h_ptrs, p * sizeof(void *), hipMemcpyHostToDevice, sl.stream)) ||
+        (e = hipMemcpyAsync(sl.d_stripes, sl.h_stripes, stripes.size() * sizeof(StripeDesc),
+                            hipMemcpyHostToDevice, sl.stream)) ||
+        (e = hipMemcpyAsync(sl.d_items, sl.h_items, items.size() * sizeof(WorkItem),
+                            hipMemcpyHostToDevice, sl.stream)))
+        return sdev("descriptor upload", e);
+    const uint32_t wpb = batch_block_threads() / 64;
+    const uint64_t want = (uint64_t)s->cus * s->cfg.waves_per_cu;
+    const uint64_t waves = std::max<uint64_t>(1, std::min<uint64_t>(want, items.size()));
+    const uint32_t blocks = (uint32_t)((waves + wpb - 1) / wpb);
+    BatchArgs a{sl.d_stripes, sl.d_items, (uint32_t)items.size(), blocks * wpb,
+                (uint32_t)s->cfg.subtiles, sl.d_counter};
+    if ((e = launch_batch(nb, s->cfg.dwords, s->cfg.rows_per_item, a, blocks, sl.stream)) ||
+        (e = hipEventRecord(sl.done, sl.stream)))
+        return sdev("batch launch", e);
+    sl.busy = true;
+    return FGF_OK;
+}
+
+void scheduler(fgf_server *s) {
+    (void)hipSetDevice(s->device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        bool any_busy = false;
+        for (Slot &sl : s->slots) {
+            if (!sl.busy) continue;
+            const hipError_t q = hipEventQuery(sl.done);
+            if (q == hipSuccess)
+                finish_slot(s, sl, 0);
+            else if (q != hipErrorNotReady)
+                finish_slot(s, sl, FGF_EDEVICE);
+            else
+                any_busy = true;
+        }
+        if (s->stats.completed + s->stats.failed == s->next_ticket) s->cv_done.notify_all();
+        Slot *free_slot = nullptr;
+        for (Slot &sl : s->slots)
+            if (!sl.busy) {
+                free_slot = &sl;
+                break;
+            }
+        bool launched = false;
+        if (free_slot && !s->queue.empty()) {
+            const int field = s->queue.front().r.field;
+            for (auto it = s->queue.begin();
+                 it != s->queue.end() && free_slot->reqs.size() < s->cfg.max_batch_stripes;) {
+                if (it->r.field != field) {
+                    ++it;
+                    continue;
+                }
+                if (s->inflight_bytes + it->bytes > s->cfg.hbm_budget_bytes) {
+                    if (free_slot->reqs.empty() && !any_busy) {
+                        // Larger than the whole budget: can never be admitted.
+                        s->latency[it->ticket] = 0;
+                        s->status[it->ticket] = FGF_ENOMEM;
+                        ++s->stats.failed;
+                        it = s->queue.erase(it);
+                        continue;
+                    }
+                    ++s->stats.admission_waits;
+                    break;  // in-order admission: wait for budget
+                }
+                PlanEntry *pe = acquire_plan(s, *it);
+                if (!pe) {
+                    s->latency[it->ticket] = 0;
+                    s->status[it->ticket] = FGF_EINVAL;
+                    ++s->stats.failed;
+                    it = s->queue.erase(it);
+                    continue;
+                }
+                s->inflight_bytes += it->bytes;
+                free_slot->bytes += it->bytes;
+                free_slot->plans.push_back(pe);
+                free_slot->reqs.push_back(std::move(*it));
+                it = s->queue.erase(it);
+            }
+            s->stats.max_inflight_bytes = std::max(s->stats.max_inflight_bytes, s->inflight_bytes);
+            if (!free_slot->reqs.empty()) {
+                // Only this thread touches slots: submissions proceed meanwhile.
+                lk.unlock();
+                const int rc = launch_slot(s, *free_slot);
+                lk.lock();
+                if (rc != FGF_OK) {
+                    finish_slot(s, *free_slot, FGF_EDEVICE);
+                } else {
+                    ++s->stats.launches;
+                    s->stats.stripes_launched += free_slot->reqs.size();
+                }
+                launched = true;
+            }
+        }
+        if (s->stop && s->queue.empty() && !any_busy && !launched) break;
+        if (!launched) {
+            // Idle or waiting on the device: poll completions at a fine grain.
+            if (any_busy)
+                s->cv_work.wait_for(lk, std::chrono::microseconds(20));
+            else
+                s->cv_work.wait_for(lk, std::chrono::milliseconds(5));
+        }
+    }
+    s->cv_done.notify_all();
+}
+
+}  // namespace
+
+extern "C" {
+
+int fgf_server_create(const fgf_server_config *cfg, fgf_server **out) {
+    if (!cfg || !out) return sfail("fgf_server_create: null argument");
+    if (cfg->slots < 1 || cfg->slots > 16) return sfail("fgf_server_create: slots must be in 1..16");
+    if (cfg->max_batch_stripes < 1) return sfail("fgf_server_create: max_batch_stripes must be >= 1");
+    if (cfg->plan_cache_entries < 1) return sfail("fgf_server_create: plan_cache_entries must be >= 1");
+    if (cfg->dwords != 4 && cfg->dwords != 8) return sfail("fgf_server_create: dwords must be 4 or 8");
+    if (cfg->rows_per_item != 4 && cfg->rows_per_item != 8)
+        return sfail("fgf_server_create: rows_per_item must be 4 or 8");
+    if (cfg->subtiles < 1 || cfg->subtiles > 64) return sfail("fgf_server_create: subtiles must be in 1..64");
+    if (cfg->waves_per_cu < 1 || cfg->waves_per_cu > 32)
+        return sfail("fgf_server_create: waves_per_cu must be in 1..32");
+    auto *s = new (std::nothrow) fgf_server;
+    if (!s) return sfail("out of host memory");
+    s->cfg = *cfg;
+    (void)hipGetDevice(&s->device);
+    (void)hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, s->device);
+    s->slots.resize(cfg->slots);
+    for (Slot &sl : s->slots) {
+        hipError_t e;
+        if ((e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking)) ||
+            (e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming)) ||
+            (e = hipMalloc((void **)&sl.d_counter, 2 * sizeof(uint32_t))) ||
+            (e = hipMemset(sl.d_counter, 0, 2 * sizeof(uint32_t)))) {
+            fgf_server_destroy(s);
+            return sdev("fgf_server_create", e);
+        }
+    }
+    s->worker = std::thread(scheduler, s);
+    *out = s;
+    return FGF_OK;
+}
+
+int fgf_server_submit(fgf_server *s, const fgf_request *r, uint64_t *ticket) {
+    if (!s || !r) return sfail("fgf_server_submit: null argument");
+    const int f = r->field;
+    if (f != FGF_GF8B && f != FGF_GF8D && f != FGF_GF16 && f != FGF_FP8 && f != FGF_FP16)
+        return sfail("fgf_server_submit: field %d is not served (byte fields and 16-bit towers)", f);
+    if (r->k < 1 || r->m < 1 || r->k + r->m > 256)
+        return sfail("fgf_server_submit: k = %u, m = %u outside 1 <= k, 1 <= m, k + m <= 256", r->k, r->m);
+    if (r->k > 64) return sfail("fgf_server_submit: k = %u exceeds 64 sources", r->k);
+    if (!r->shards) return sfail("fgf_server_submit: shards is null");
+    if (r->row_len == 0 || r->row_len % 16)
+        return sfail("fgf_server_submit: row_len %zu is not a nonzero multiple of 16", r->row_len);
+    for (uint32_t i = 0; i < r->k + r->m; ++i)
+        if ((uintptr_t)r->shards[i] & 15u) return sfail("fgf_server_submit: shard %u is not 16-byte aligned", i);
+    Req q;
+    q.r = *r;
+    q.shards.assign(r->shards, r->shards + r->k + r->m);
+    if (r->reconstruct) {
+        if (r->nerased < 1 || r->nerased > r->m || !r->erased)
+            return sfail("fgf_server_submit: reconstruct needs 1..m erased shards");
+        q.erased.assign(r->erased, r->erased + r->nerased);
+        std::sort(q.erased.begin(), q.erased.end());
+        for (size_t i = 0; i < q.erased.size(); ++i)
+            if (q.erased[i] >= r->k + r->m || (i && q.erased[i] == q.erased[i - 1]))
+                return sfail("fgf_server_submit: erased shard list is invalid");
+    }
+    q.r.shards = nullptr;
+    q.r.erased = nullptr;
+    q.bytes = (uint64_t)(r->k + r->m) * r->row_len;
+    q.submitted = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        q.ticket = s->next_ticket++;
+        s->latency.push_back(-1.0);
+        s->status.push_back(0);
+        ++s->stats.submitted;
+        if (ticket) *ticket = q.ticket;
+        s->queue.push_back(std::move(q));
+    }
+    s->cv_work.notify_one();
+    return FGF_OK;
+}
+
+int fgf_server_drain(fgf_server *s) {
+    if (!s) return sfail("fgf_server_drain: null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return s->stats.completed + s->stats.failed == s->next_ticket; });
+    return s->stats.failed ? FGF_EDEVICE : FGF_OK;
+}
+
+int fgf_server_result(fgf_server *s, uint64_t ticket, double *latency_s, int *status) {
+    if (!s) return sfail("fgf_server_result: null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (ticket >= s->latency.size()) return sfail("fgf_server_result: unknown ticket %llu", (unsigned long long)ticket);
+    if (latency_s) *latency_s = s->latency[ticket];
+    if (status) *status = s->status[ticket];
+    return FGF_OK;
+}
+
+int fgf_server_stats_get(fgf_server *s, fgf_server_stats *out) {
+    if (!s || !out) return sfail("fgf_server_stats_get: null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = s->stats;
+    return FGF_OK;
+}
+
+int fgf_server_replay(fgf_server *s, const fgf_request *reqs, const double *arrival, size_t n,
+                      double *latency_s, double *wall_s) {
+    if (!s || (n && (!reqs || !arrival))) return sfail("fgf_server_replay: null argument");
+    std::vector<uint64_t> tickets(n);
+    std::vector<double> lateness(n, 0.0);
+    int rc = FGF_OK;
+    const Clock::time_point t0 = Clock::now();
+    std::thread client([&] {
+        for (size_t i = 0; i < n; ++i) {
+            const Clock::time_point due =
+                t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(arrival[i]));
+            while (Clock::now() < due) {
+            }
+            lateness[i] = std::chrono::duration<double>(Clock::now() - due).count();
+            const int r = fgf_server_submit(s, &reqs[i], &tickets[i]);
+            if (r != FGF_OK) {
+                rc = r;
+                tickets[i] = UINT64_MAX;
+            }
+        }
+    });
+    client.join();
+    const int d = fgf_server_drain(s);
+    if (wall_s) *wall_s = std::chrono::duration<double>(Clock::now() - t0).count();
+    if (latency_s) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (size_t i = 0; i < n; ++i) {
+            if (tickets[i] == UINT64_MAX) {
+                latency_s[i] = -1;
+                continue;
+            }
+            // From the scheduled arrival: includes any lateness of the client.
+            latency_s[i] = s->latency[tickets[i]] + lateness[i];
+        }
+    }
+    return rc != FGF_OK ? rc : d;
+}
+
+void fgf_server_destroy(fgf_server *s) {
+    if (!s) return;
+    if (s->worker.joinable()) {
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->stop = true;
+        }
+        s->cv_work.notify_all();
+        s->worker.join();
+    }
+    for (Slot &sl : s->slots) {
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+        if (sl.d_stripes) (void)hipFree(sl.d_stripes);
+        if (sl.h_stripes) (void)hipHostFree(sl.h_stripes);
+        if (sl.d_items) (void)hipFree(sl.d_items);
+        if (sl.h_items) (void)hipHostFree(sl.h_items);
+        if (sl.d_ptrs) (void)hipFree(sl.d_ptrs);
+        if (sl.h_ptrs) (void)hipHostFree(sl.h_ptrs);
+        if (sl.d_counter) (void)hipFree(sl.d_counter);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+    }
+    for (auto &kv : s->cache) fgf_plan_free(kv.second.plan);
+    delete s;
+}
+
+}  // extern "C"

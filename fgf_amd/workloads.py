@@ -114,6 +114,99 @@ def materialize(field: Field, specs, seed: int, device="cuda"):
     return pool, saved
 
 
+@dataclass
+class ServeSpec:
+    field: Field
+    k: int
+    m: int
+    row_len: int
+    erased: list
+    offset: int = 0
+
+    @property
+    def codeword_bytes(self) -> int:
+        return (self.k + self.m) * self.row_len
+
+    @property
+    def out_shards(self) -> list[int]:
+        return self.erased if self.erased else list(range(self.k, self.k + self.m))
+
+
+def draw_serving_mix(fields_weights, n: int, seed: int, n_matrices: int = 256, kmin=4, kmax=32,
+                     mmin=2, mmax=8, lmin=4096, lmax=1 << 20, align=64,
+                     reconstruct_frac=0.5) -> list[ServeSpec]:
+    """Requests over `n_matrices` distinct coefficient matrices: each matrix
+    is (field, k, m, erased set), drawn once; requests pick a matrix uniformly
+    and a shard length log-uniform on [lmin, lmax]."""
+    rng = np.random.default_rng(seed)
+    fields = [f for f, _ in fields_weights]
+    w = np.array([x for _, x in fields_weights], dtype=float)
+    w /= w.sum()
+    templates, seen = [], set()
+    while len(templates) < n_matrices:
+        f = fields[int(rng.choice(len(fields), p=w))]
+        k = int(rng.integers(kmin, kmax + 1))
+        m = int(rng.integers(mmin, mmax + 1))
+        erased = []
+        if rng.random() < reconstruct_frac:
+            e = int(rng.integers(1, m + 1))
+            erased = sorted(int(x) for x in rng.choice(k + m, e, replace=False))
+        key = (f.id, k, m, tuple(erased))
+        if key not in seen:
+            seen.add(key)
+            templates.append(key[:3] + (erased,))
+    specs, off = [], 0
+    by_id = {f.id: f for f in fields}
+    for _ in range(n):
+        fid, k, m, erased = templates[int(rng.integers(0, len(templates)))]
+        L = int(math.exp(rng.uniform(math.log(lmin), math.log(lmax))))
+        L = max(lmin, L // align * align)
+        sp = ServeSpec(by_id[fid], k, m, L, list(erased), off)
+        specs.append(sp)
+        off += (sp.codeword_bytes + 255) // 256 * 256
+    return specs
+
+
+def materialize_serving(specs, seed: int, device="cuda"):
+    """Pool with valid codewords (parity by the per-call path). Returns
+    (pool, expected): expected[i] is the bytes request i must produce (its
+    parity for an encode, the erased shards for a reconstruct), and those
+    output shards are cleared in the pool."""
+    import torch
+    total = specs[-1].offset + (specs[-1].codeword_bytes + 255) // 256 * 256
+    pool = torch.empty(total, dtype=torch.uint8, device=device)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+    for s in range(0, total, 1 << 28):
+        n = min(1 << 28, total - s)
+        pool[s:s + n] = torch.randint(0, 256, (n,), dtype=torch.uint8, device=device, generator=gen)
+    plans, expected = {}, []
+    for sp in specs:
+        L = sp.row_len
+        key = (sp.field.id, sp.k, sp.m)
+        if key not in plans:
+            plans[key] = ops.Plan.matrix(sp.field, sp.k, sp.m,
+                                         codec.encode_plan_values(sp.field, sp.k, sp.m))
+        rows = pool[sp.offset + sp.k * L: sp.offset + (sp.k + sp.m) * L]
+        ops.dot_product_matrix_with(sp.field, rows, L, sp.m, plans[key],
+                                    [serve_shard(pool, sp, i) for i in range(sp.k)])
+        expected.append(torch.cat([serve_shard(pool, sp, i) for i in sp.out_shards]).clone())
+        for i in sp.out_shards:
+            serve_shard(pool, sp, i).zero_()
+    torch.cuda.synchronize()
+    return pool, expected
+
+
+def serve_shard(pool, sp: ServeSpec, i: int):
+    return pool[sp.offset + i * sp.row_len: sp.offset + (i + 1) * sp.row_len]
+
+
+def serve_request(pool, sp: ServeSpec):
+    from .serve import Request
+    return Request(sp.field, sp.k, sp.m, [serve_shard(pool, sp, i) for i in range(sp.k + sp.m)],
+                   sp.row_len, list(sp.erased))
+
+
 def stripe_buffers(pool, sp: StripeSpec):
     L = sp.row_len
     shard = lambda i: pool[sp.offset + i * L: sp.offset + (i + 1) * L]

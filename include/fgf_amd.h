@@ -241,6 +241,55 @@ uint64_t fgf_batch_bytes(const fgf_batch *batch);
 void fgf_batch_free(fgf_batch *batch);
 
 /* ------------------------------------------------------------------------
+ * Erasure-coding service. No counterpart in the reference (a library, not a
+ * server): a request loop over the batched kernel. Requests encode or
+ * reconstruct one stripe of a systematic code [I_k ; C], C the m x k Cauchy
+ * matrix 1/(i + (m + j)), over GF(2^8)/0x11D, GF(2^8)/0x11B, GF(2^16) or
+ * FanPaar8/16. A scheduler thread admits queued requests while the bytes of
+ * admitted-but-unfinished requests stay within the HBM budget, packs them
+ * into one persistent launch per free slot (each slot its own stream, so
+ * batches overlap on the device), and keeps coefficient plans — encode
+ * matrices and inverted decode matrices — in a device LRU cache.
+ * ---------------------------------------------------------------------- */
+typedef struct fgf_server fgf_server;
+typedef struct fgf_server_config {
+    uint64_t hbm_budget_bytes;     /* admitted working-set bytes, (k + m) * row_len each */
+    uint32_t slots;                /* concurrent launches in flight (streams), 1..16 */
+    uint32_t max_batch_stripes;    /* requests per launch, >= 1 */
+    uint32_t plan_cache_entries;   /* device plans kept, >= 1 */
+    int dwords, rows_per_item, subtiles, waves_per_cu; /* kernel shape, as fgf_batch_configure */
+} fgf_server_config;
+typedef struct fgf_request {
+    fgf_field field;
+    int reconstruct;               /* 0: write the m parity shards; 1: rebuild `erased` */
+    uint32_t k, m;
+    uint8_t *const *shards;        /* k + m device pointers: data shards then parity shards */
+    size_t row_len;                /* bytes per shard, multiple of 16, pointers 16-byte aligned */
+    const uint32_t *erased;        /* reconstruct: shard indices to rebuild (1..m of them) */
+    uint32_t nerased;
+} fgf_request;
+typedef struct fgf_server_stats {
+    uint64_t submitted, completed, failed, bytes, launches, stripes_launched;
+    uint64_t admission_waits, plan_hits, plan_misses, plan_evictions;
+    uint64_t max_inflight_bytes;
+} fgf_server_stats;
+int fgf_server_create(const fgf_server_config *config, fgf_server **out);
+/* Thread-safe; validates and queues. `ticket` numbers requests 0, 1, 2, ... */
+int fgf_server_submit(fgf_server *server, const fgf_request *request, uint64_t *ticket);
+/* Blocks until every submitted request has completed. */
+int fgf_server_drain(fgf_server *server);
+/* Seconds from submission to observed completion of a completed ticket
+ * (negative while pending); status 0 ok, negative on failure. */
+int fgf_server_result(fgf_server *server, uint64_t ticket, double *latency_s, int *status);
+int fgf_server_stats_get(fgf_server *server, fgf_server_stats *out);
+/* Open-loop driver: submits requests[i] at arrival_s[i] seconds after the
+ * call starts (a client thread spinning on the clock), waits for all, and
+ * writes per-request latency from the scheduled arrival to completion. */
+int fgf_server_replay(fgf_server *server, const fgf_request *requests, const double *arrival_s,
+                      size_t n, double *latency_s, double *wall_s);
+void fgf_server_destroy(fgf_server *server);
+
+/* ------------------------------------------------------------------------
  * Element packing (src/ops.rs:963-1017), host memory only.
  * ---------------------------------------------------------------------- */
 /* pack: elems (raw integers) -> little-endian bytes. */

@@ -269,3 +269,11 @@ def test_zero_nrows_and_empty_inputs_are_host_side_no_ops():
     ops.mul_add(Gf8B, z(16), 0, z(16))
     ops.mul_assign(Gf16, e, 7)
     ops.mul_add_matrix_scattered(Gf8B, z(32), 16, [], [])
+
+
+def test_server_config_is_validated_on_the_host():
+    from fgf_amd.serve import Server
+    with pytest.raises(Panic, match="slots must be in 1..16"):
+        Server(slots=0)
+    with pytest.raises(Panic, match="dwords must be 4 or 8"):
+        Server(dwords=3)
