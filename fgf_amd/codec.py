@@ -70,6 +70,29 @@ def product_table(field: Field):
     return _PRODUCT_TABLES[field.id]
 
 
+def host_dot_product_matrix(field: Field, values: Sequence[int], srcs, nrows: int):
+    """rows[j] = sum_t values[t*nrows + j] * srcs[t] on the host, from the
+    library's scalar field arithmetic (numpy byte arrays in and out; for spot
+    checks of device results on small slices)."""
+    import numpy as np
+    B = field.BYTES
+    n = len(srcs[0])
+    out = np.zeros((nrows, n), dtype=np.uint8)
+    if B == 1:
+        t = product_table(field)
+        for ti, s in enumerate(srcs):
+            for j in range(nrows):
+                out[j] ^= t[values[ti * nrows + j], s]
+        return out
+    for ti, s in enumerate(srcs):
+        elems = [int.from_bytes(s[e * B:(e + 1) * B].tobytes(), "little") for e in range(n // B)]
+        for j in range(nrows):
+            c = values[ti * nrows + j]
+            prod = b"".join(field.mul(c, x).to_bytes(B, "little") for x in elems)
+            out[j] ^= np.frombuffer(prod, dtype=np.uint8)
+    return out
+
+
 def matmul(field: Field, a: Sequence[Sequence[int]], b: Sequence[Sequence[int]]) -> list[list[int]]:
     if field.BYTES == 1:
         import numpy as np
