@@ -47,10 +47,11 @@ __global__ __launch_bounds__(BATCH_BLOCK) void k_batch(const BatchArgs a) {
         const uint32_t mrows = min((uint32_t)M, s.m - j0);
         uint64_t tile = (uint64_t)w.chunk * a.subtiles * CHUNK;
         for (uint32_t c = 0; c < a.subtiles && tile < s.row_len; ++c, tile += CHUNK)
-            core::matrix_tile<B, M, D, true>(s.srcs, s.rows + j0,
-                                             s.tab + (size_t)j0 * s.tab_stride_j, s.tab_stride_t,
-                                             s.tab_stride_j, s.k, s.overwrite != 0, 0, s.row_len,
-                                             tile, lane, mrows);
+            core::matrix_tile<B, M, D, true>(
+                s.srcs, s.rows + j0,
+                core::GlobalTab<B>{s.tab + (size_t)j0 * s.tab_stride_j, s.tab_stride_t,
+                                   s.tab_stride_j},
+                s.k, s.overwrite != 0, 0, s.row_len, tile, lane, mrows);
         it = __builtin_amdgcn_readfirstlane(claim);
     }
     // Retire: the last wave out resets the queue for the next launch.

@@ -63,6 +63,9 @@ struct MatrixArgs {
 
 constexpr int BLOCK = 256;
 
+// Byte-field matrix kernels read their T0/T1 tables from LDS (see k_matrix).
+constexpr bool lds_tables(int nb) { return nb == 1; }
+
 // Block index -> (column tile, row group). Blocks are dealt to the XCDs round
 // robin, so with several row groups the blocks b, b + 8, b + 16, ... (one XCD)
 // visit every row group of one column tile before moving on: the tile's
@@ -86,6 +89,9 @@ __device__ __forceinline__ bool map_block(const MatrixArgs &a, uint32_t &ct, uin
 
 // Vector kernel: every pointer + col0 is 16-byte aligned and ncols is a
 // multiple of 16.
+//
+// Byte fields stage the block's T0/T1 tables in LDS (k*M 16-byte records of
+// dynamic shared memory, at most 16 KiB) before any wave starts its tile.
 template <int B, int M, int D, bool OW>
 __global__ __launch_bounds__(BLOCK) void k_matrix(const MatrixArgs a) {
     uint32_t ct, rg;
@@ -93,10 +99,19 @@ __global__ __launch_bounds__(BLOCK) void k_matrix(const MatrixArgs a) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint64_t tile = ((uint64_t)ct * (BLOCK / 64) + wave) * (256u * D);
-    if (tile >= a.ncols) return;
     const uint32_t j0 = rg * M;
-    matrix_tile<B, M, D>(a.src, a.dst + j0, a.tab + (size_t)j0 * a.tab_stride_j, a.tab_stride_t,
-                         a.tab_stride_j, a.k, OW, a.col0, a.ncols, tile, lane);
+    const uint32_t *tab = a.tab + (size_t)j0 * a.tab_stride_j;
+    if constexpr (lds_tables(B)) {
+        extern __shared__ uint4 s_lt[];
+        stage_lds_table(s_lt, tab, a.k, M, a.tab_stride_t, a.tab_stride_j);
+        if (tile >= a.ncols) return;
+        matrix_tile<B, M, D>(a.src, a.dst + j0, LdsTab{s_lt, tab, a.tab_stride_t, a.tab_stride_j, M},
+                             a.k, OW, a.col0, a.ncols, tile, lane);
+    } else {
+        if (tile >= a.ncols) return;
+        matrix_tile<B, M, D>(a.src, a.dst + j0, GlobalTab<B>{tab, a.tab_stride_t, a.tab_stride_j},
+                             a.k, OW, a.col0, a.ncols, tile, lane);
+    }
 }
 
 // Generic kernel for heads, tails and unaligned buffers: one thread per group
@@ -214,7 +229,8 @@ hipError_t launch_segment(const MatrixJob &job, MatrixArgs a, size_t k0, size_t 
         const uint64_t blocks = a.ngroups == 1
                                     ? a.ncoltiles
                                     : (uint64_t)((a.ncoltiles + 7) / 8) * 8u * a.ngroups;
-        hipLaunchKernelGGL(pick_vector(nb, mt, ow), dim3((uint32_t)blocks), dim3(BLOCK), 0,
+        const size_t lds = lds_tables(nb) ? kn * (size_t)mt * sizeof(uint4) : 0;
+        hipLaunchKernelGGL(pick_vector(nb, mt, ow), dim3((uint32_t)blocks), dim3(BLOCK), lds,
                            job.stream, a);
     } else {
         a.ngroups = 1;

@@ -161,10 +161,62 @@ __device__ __forceinline__ void load_source(uint32_t (&x)[D], SrcPtrs src, uint3
     }
 }
 
-// Folds S (1 or 2) sources (dwords x0, x1) into the row accumulators. The
-// table of source s, row j, entry (p, q) starts at
-// tt + s*stride_t + j*stride_j + (p*B + q)*TAB_DW. Each (row, plane) folds
-// all of its 3*B*S lookups with v_bitop3. Two loop orders:
+// Coefficient-table views. An entry is the five dwords of one byte constant
+// (T0 lo/hi, T1 lo/hi, T2); the entry of source t + s, row j, plane (p, q)
+// is at tt + s*stride_t + j*stride_j + (p*B + q)*TAB_DW in the plan's table.
+//
+// GlobalTab reads entries with wave-uniform addresses (scalar loads).
+// v_perm_b32 takes one scalar operand, so each T0/T1 pair then costs a v_mov
+// per source and row.
+template <int B>
+struct GlobalTab {
+    const uint32_t *__restrict__ tt;   // source t, row 0
+    uint32_t stride_t, stride_j;
+    __device__ __forceinline__ GlobalTab at(uint32_t dt) const {
+        return {tt + __builtin_amdgcn_readfirstlane(dt * stride_t), stride_t, stride_j};
+    }
+    __device__ __forceinline__ void fold(Fold &f, int s, int j, int p, int q, const Sel &sel) const {
+        cmul_fold(f, tt + s * stride_t + j * stride_j + (p * B + q) * TAB_DW, sel);
+    }
+};
+
+// LdsTab (byte fields): the T0/T1 pairs of the block's rows are staged in LDS
+// as one 16-byte record per (source, row) and read back with broadcast
+// ds_read_b128 straight into the VGPRs v_perm needs; T2 (both halves the same
+// dword) stays a scalar operand.
+struct LdsTab {
+    const uint4 *lt;                   // LDS records of source t, row 0
+    const uint32_t *__restrict__ tt;   // plan table of source t, row 0 (for T2)
+    uint32_t stride_t, stride_j, m;    // m: rows per source in the LDS records
+    __device__ __forceinline__ LdsTab at(uint32_t dt) const {
+        return {lt + dt * m, tt + __builtin_amdgcn_readfirstlane(dt * stride_t), stride_t,
+                stride_j, m};
+    }
+    __device__ __forceinline__ void fold(Fold &f, int s, int j, int, int, const Sel &sel) const {
+        const uint4 e = lt[s * m + j];
+        const uint32_t t2 = tt[s * stride_t + j * stride_j + 4];
+        f.add(vperm(e.y, e.x, sel.s0));
+        f.add(vperm(e.w, e.z, sel.s1));
+        f.add(vperm(t2, t2, sel.s2));
+    }
+};
+
+// Stages the T0/T1 pairs of rows [0, m) of sources [0, k) into LDS. Every
+// thread of the block must call it (it ends with a barrier).
+__device__ __forceinline__ void stage_lds_table(uint4 *lt, const uint32_t *__restrict__ tab,
+                                                uint32_t k, uint32_t m, uint32_t stride_t,
+                                                uint32_t stride_j) {
+    for (uint32_t i = threadIdx.x; i < k * m; i += blockDim.x) {
+        const uint32_t t = i / m, j = i - t * m;
+        const uint32_t *e = tab + (size_t)t * stride_t + (size_t)j * stride_j;
+        lt[i] = make_uint4(e[0], e[1], e[2], e[3]);
+    }
+    __syncthreads();
+}
+
+// Folds S (1 or 2) sources (dwords x0, x1) into the row accumulators, with
+// the tables of sources t and t + 1 seen through `tb`. Each (row, plane)
+// folds all of its 3*B*S lookups with v_bitop3. Two loop orders:
 //  - dword-outer: one dword's selectors live, every table of the step held
 //    in SGPRs (S*M*5 dwords) -- byte fields with few rows;
 //  - row-outer: all selectors of the step live, one row's tables at a time.
@@ -173,10 +225,9 @@ __device__ __forceinline__ void load_source(uint32_t (&x)[D], SrcPtrs src, uint3
 template <int B, int M>
 constexpr bool dword_outer() { return B == 1 && M <= 6; }
 
-template <int B, int M, int D, int S, bool GUARD>
+template <int B, int M, int D, int S, bool GUARD, typename Tab>
 __device__ __forceinline__ void consume(uint32_t (&acc)[M][D], const uint32_t *x0,
-                                        const uint32_t *x1, const uint32_t *__restrict__ tt,
-                                        uint32_t stride_t, uint32_t stride_j, uint32_t mrows) {
+                                        const uint32_t *x1, const Tab &tb, uint32_t mrows) {
     constexpr int G = D / B;
     const uint32_t *xs[2] = {x0, x1};
     if constexpr (!GUARD && dword_outer<B, M>()) {
@@ -198,9 +249,7 @@ __device__ __forceinline__ void consume(uint32_t (&acc)[M][D], const uint32_t *x
 #pragma unroll
                     for (int s = 0; s < S; ++s)
 #pragma unroll
-                        for (int q = 0; q < B; ++q)
-                            cmul_fold(f, tt + s * stride_t + j * stride_j + (p * B + q) * TAB_DW,
-                                      sel[s][q]);
+                        for (int q = 0; q < B; ++q) tb.fold(f, s, j, p, q, sel[s][q]);
                     acc[j][g * B + p] = f.done();
                 }
             }
@@ -227,9 +276,7 @@ __device__ __forceinline__ void consume(uint32_t (&acc)[M][D], const uint32_t *x
 #pragma unroll
                     for (int s = 0; s < S; ++s)
 #pragma unroll
-                        for (int q = 0; q < B; ++q)
-                            cmul_fold(f, tt + s * stride_t + j * stride_j + (p * B + q) * TAB_DW,
-                                      sel[s][g * B + q]);
+                        for (int q = 0; q < B; ++q) tb.fold(f, s, j, p, q, sel[s][g * B + q]);
                     acc[j][g * B + p] = f.done();
                 }
             }
@@ -238,21 +285,20 @@ __device__ __forceinline__ void consume(uint32_t (&acc)[M][D], const uint32_t *x
 }
 
 // Folds the sources t (and t + 1 when present) held in x.
-template <int B, int M, int D, bool GUARD>
+template <int B, int M, int D, bool GUARD, typename Tab>
 __device__ __forceinline__ void consume_step(uint32_t (&acc)[M][D], const uint32_t (&x)[2][D],
-                                             const uint32_t *__restrict__ tab, uint32_t t,
-                                             uint32_t k, uint32_t stride_t, uint32_t stride_j,
+                                             const Tab &tab, uint32_t t, uint32_t k,
                                              uint32_t mrows) {
-    const uint32_t *tt = tab + __builtin_amdgcn_readfirstlane(t * stride_t);
+    const Tab tb = tab.at(t);
     if (t + 1 < k) {
         if constexpr (B == 1) {
-            consume<B, M, D, 2, GUARD>(acc, x[0], x[1], tt, stride_t, stride_j, mrows);
+            consume<B, M, D, 2, GUARD>(acc, x[0], x[1], tb, mrows);
         } else {
-            consume<B, M, D, 1, GUARD>(acc, x[0], x[0], tt, stride_t, stride_j, mrows);
-            consume<B, M, D, 1, GUARD>(acc, x[1], x[1], tt + stride_t, stride_t, stride_j, mrows);
+            consume<B, M, D, 1, GUARD>(acc, x[0], x[0], tb, mrows);
+            consume<B, M, D, 1, GUARD>(acc, x[1], x[1], tab.at(t + 1), mrows);
         }
     } else {
-        consume<B, M, D, 1, GUARD>(acc, x[0], x[0], tt, stride_t, stride_j, mrows);
+        consume<B, M, D, 1, GUARD>(acc, x[0], x[0], tb, mrows);
     }
 }
 
@@ -265,12 +311,11 @@ __device__ __forceinline__ void consume_step(uint32_t (&acc)[M][D], const uint32
 // (seed from the destination). Sources are consumed in pairs from two
 // ping-pong register buffers: the next pair's loads are in flight while the
 // current pair is multiplied.
-template <int B, int M, int D, bool GUARD = false, typename SrcPtrs, typename DstPtrs>
-__device__ __forceinline__ void matrix_tile(SrcPtrs src, DstPtrs dst,
-                                            const uint32_t *__restrict__ tab, uint32_t stride_t,
-                                            uint32_t stride_j, uint32_t k, bool ow,
-                                            uint64_t col0, uint64_t ncols, uint64_t tile,
-                                            uint32_t lane, uint32_t mrows = M) {
+template <int B, int M, int D, bool GUARD = false, typename SrcPtrs, typename DstPtrs,
+          typename Tab>
+__device__ __forceinline__ void matrix_tile(SrcPtrs src, DstPtrs dst, const Tab &tab, uint32_t k,
+                                            bool ow, uint64_t col0, uint64_t ncols,
+                                            uint64_t tile, uint32_t lane, uint32_t mrows = M) {
     constexpr int P = D / 4;
     constexpr int G = D / B;
     static_assert(D % 4 == 0 && D % B == 0, "tile shape");
@@ -303,12 +348,12 @@ __device__ __forceinline__ void matrix_tile(SrcPtrs src, DstPtrs dst,
     for (uint32_t t = 0;;) {
         if (t + 2 < k) load_source<D>(xb[0], src, t + 2, base, tl);
         if (t + 3 < k) load_source<D>(xb[1], src, t + 3, base, tl);
-        consume_step<B, M, D, GUARD>(acc, xa, tab, t, k, stride_t, stride_j, mrows);
+        consume_step<B, M, D, GUARD>(acc, xa, tab, t, k, mrows);
         t += 2;
         if (t >= k) break;
         if (t + 2 < k) load_source<D>(xa[0], src, t + 2, base, tl);
         if (t + 3 < k) load_source<D>(xa[1], src, t + 3, base, tl);
-        consume_step<B, M, D, GUARD>(acc, xb, tab, t, k, stride_t, stride_j, mrows);
+        consume_step<B, M, D, GUARD>(acc, xb, tab, t, k, mrows);
         t += 2;
         if (t >= k) break;
     }
