@@ -63,8 +63,17 @@ struct MatrixArgs {
 
 constexpr int BLOCK = 256;
 
-// Byte-field matrix kernels read their T0/T1 tables from LDS (see k_matrix).
-constexpr bool lds_tables(int nb) { return nb == 1; }
+// Matrix kernels for B <= 4 read their tables from LDS (see k_matrix):
+// k*M*B*B records per block, kept within 64 KiB by the source chunk.
+// GF(2^64) keeps scalar tables: its 2.5 KiB of records per source would cut
+// the chunk to 25 sources and the occupancy to two blocks per CU, which
+// measured slower than the v_mov copies they save.
+constexpr bool lds_tables(int nb) { return nb <= 4; }
+constexpr size_t LDS_TABLE_BYTES = 64 * 1024;
+// T2 dwords go to LDS too when a block's B*B*M scalars per source would crowd
+// the SGPRs (wide fields, and byte fields with more than 8 rows).
+constexpr bool t2_lds(int nb, int m) { return nb > 1 || m > 8; }
+constexpr size_t lds_record_bytes(int nb, int m) { return (size_t)nb * nb * (t2_lds(nb, m) ? 20 : 16); }
 
 // Block index -> (column tile, row group). Blocks are dealt to the XCDs round
 // robin, so with several row groups the blocks b, b + 8, b + 16, ... (one XCD)
@@ -90,8 +99,8 @@ __device__ __forceinline__ bool map_block(const MatrixArgs &a, uint32_t &ct, uin
 // Vector kernel: every pointer + col0 is 16-byte aligned and ncols is a
 // multiple of 16.
 //
-// Byte fields stage the block's T0/T1 tables in LDS (k*M 16-byte records of
-// dynamic shared memory, at most 16 KiB) before any wave starts its tile.
+// The block first stages its rows' tables in LDS (dynamic shared memory, at
+// most LDS_TABLE_BYTES), then every wave runs its tile.
 template <int B, int M, int D, bool OW>
 __global__ __launch_bounds__(BLOCK) void k_matrix(const MatrixArgs a) {
     uint32_t ct, rg;
@@ -103,10 +112,10 @@ __global__ __launch_bounds__(BLOCK) void k_matrix(const MatrixArgs a) {
     const uint32_t *tab = a.tab + (size_t)j0 * a.tab_stride_j;
     if constexpr (lds_tables(B)) {
         extern __shared__ uint4 s_lt[];
-        stage_lds_table(s_lt, tab, a.k, M, a.tab_stride_t, a.tab_stride_j);
+        const auto tb = stage_lds_table<B, t2_lds(B, M)>(s_lt, tab, a.k, M, a.tab_stride_t,
+                                                      a.tab_stride_j);
         if (tile >= a.ncols) return;
-        matrix_tile<B, M, D>(a.src, a.dst + j0, LdsTab{s_lt, tab, a.tab_stride_t, a.tab_stride_j, M},
-                             a.k, OW, a.col0, a.ncols, tile, lane);
+        matrix_tile<B, M, D>(a.src, a.dst + j0, tb, a.k, OW, a.col0, a.ncols, tile, lane);
     } else {
         if (tile >= a.ncols) return;
         matrix_tile<B, M, D>(a.src, a.dst + j0, GlobalTab<B>{tab, a.tab_stride_t, a.tab_stride_j},
@@ -162,6 +171,12 @@ using MatrixFn = void (*)(MatrixArgs);
 
 // Rows per block for each element width, bounded by the accumulator budget.
 constexpr int max_rows(int nb) { return nb == 1 ? 16 : nb == 2 ? 8 : nb == 4 ? 4 : 2; }
+// Sources per launch: KMAX, or fewer when a full row group's LDS tables would
+// exceed LDS_TABLE_BYTES.
+constexpr size_t k_chunk(int nb) {
+    return lds_tables(nb) ? std::min<size_t>(KMAX, LDS_TABLE_BYTES / (max_rows(nb) * lds_record_bytes(nb, max_rows(nb))))
+                          : (size_t)KMAX;
+}
 // Byte fields: 16 bytes per lane per source (1 KiB per wave per stream), the
 // granularity at which the encode mix streams fastest; wider elements keep
 // 32 bytes so a lane holds whole 4-element groups.
@@ -229,7 +244,7 @@ hipError_t launch_segment(const MatrixJob &job, MatrixArgs a, size_t k0, size_t 
         const uint64_t blocks = a.ngroups == 1
                                     ? a.ncoltiles
                                     : (uint64_t)((a.ncoltiles + 7) / 8) * 8u * a.ngroups;
-        const size_t lds = lds_tables(nb) ? kn * (size_t)mt * sizeof(uint4) : 0;
+        const size_t lds = lds_tables(nb) ? kn * (size_t)mt * lds_record_bytes(nb, mt) : 0;
         hipLaunchKernelGGL(pick_vector(nb, mt, ow), dim3((uint32_t)blocks), dim3(BLOCK), lds,
                            job.stream, a);
     } else {
@@ -260,8 +275,9 @@ hipError_t launch_matrix(const MatrixJob &job) {
     const uint64_t tail0 = head + body, tail = job.row_len - tail0;
 
     MatrixArgs a{};
-    for (size_t k0 = 0; k0 < job.k; k0 += KMAX) {
-        const size_t kn = std::min<size_t>(KMAX, job.k - k0);
+    const size_t kc = k_chunk(nb);
+    for (size_t k0 = 0; k0 < job.k; k0 += kc) {
+        const size_t kn = std::min<size_t>(kc, job.k - k0);
         const bool ow = job.overwrite && k0 == 0;
         for (size_t j0 = 0; j0 < job.m; j0 += MMAX) {
             const size_t mn = std::min<size_t>(MMAX, job.m - j0);

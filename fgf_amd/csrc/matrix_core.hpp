@@ -180,38 +180,58 @@ struct GlobalTab {
     }
 };
 
-// LdsTab (byte fields): the T0/T1 pairs of the block's rows are staged in LDS
-// as one 16-byte record per (source, row) and read back with broadcast
-// ds_read_b128 straight into the VGPRs v_perm needs; T2 (both halves the same
-// dword) stays a scalar operand.
+// LdsTab: the tables of the block's rows are staged in LDS, one 16-byte
+// record {T0 lo, T0 hi, T1 lo, T1 hi} per (source, row, plane pair), read
+// back with broadcast ds_read_b128 straight into the VGPRs v_perm needs. T2
+// (one dword, both v_perm halves) stays a scalar operand for byte fields
+// (T2LDS false); wider fields, whose B*B entries per (source, row) would
+// crowd the SGPRs, read it from a second LDS array of dwords after the
+// records.
+template <int B, bool T2LDS>
 struct LdsTab {
     const uint4 *lt;                   // LDS records of source t, row 0
-    const uint32_t *__restrict__ tt;   // plan table of source t, row 0 (for T2)
+    const uint32_t *l2;                // LDS T2 dwords of source t, row 0 (T2LDS)
+    const uint32_t *__restrict__ tt;   // plan table of source t, row 0 (T2 when !T2LDS)
     uint32_t stride_t, stride_j, m;    // m: rows per source in the LDS records
     __device__ __forceinline__ LdsTab at(uint32_t dt) const {
-        return {lt + dt * m, tt + __builtin_amdgcn_readfirstlane(dt * stride_t), stride_t,
-                stride_j, m};
+        return {lt + dt * m * (B * B), l2 + dt * m * (B * B),
+                tt + __builtin_amdgcn_readfirstlane(dt * stride_t), stride_t, stride_j, m};
     }
-    __device__ __forceinline__ void fold(Fold &f, int s, int j, int, int, const Sel &sel) const {
-        const uint4 e = lt[s * m + j];
-        const uint32_t t2 = tt[s * stride_t + j * stride_j + 4];
+    __device__ __forceinline__ void fold(Fold &f, int s, int j, int p, int q, const Sel &sel) const {
+        const int r = (s * m + j) * (B * B) + p * B + q;
+        const uint4 e = lt[r];
+        const uint32_t t2 = T2LDS ? l2[r] : tt[s * stride_t + j * stride_j + (p * B + q) * TAB_DW + 4];
         f.add(vperm(e.y, e.x, sel.s0));
         f.add(vperm(e.w, e.z, sel.s1));
         f.add(vperm(t2, t2, sel.s2));
     }
 };
 
-// Stages the T0/T1 pairs of rows [0, m) of sources [0, k) into LDS. Every
-// thread of the block must call it (it ends with a barrier).
-__device__ __forceinline__ void stage_lds_table(uint4 *lt, const uint32_t *__restrict__ tab,
-                                                uint32_t k, uint32_t m, uint32_t stride_t,
-                                                uint32_t stride_j) {
-    for (uint32_t i = threadIdx.x; i < k * m; i += blockDim.x) {
-        const uint32_t t = i / m, j = i - t * m;
-        const uint32_t *e = tab + (size_t)t * stride_t + (size_t)j * stride_j;
+// LDS bytes that stage_lds_table uses for k sources and m rows.
+template <int B, bool T2LDS>
+constexpr size_t lds_table_bytes(size_t k, size_t m) {
+    return k * m * B * B * (T2LDS ? 20u : 16u);
+}
+
+// Stages the tables of rows [0, m) of sources [0, k) into LDS: record
+// (t*m + j)*B*B + p*B + q, then (T2LDS) the T2 dwords in the same order.
+// Every thread of the block must call it (it ends with a barrier).
+template <int B, bool T2LDS>
+__device__ __forceinline__ LdsTab<B, T2LDS> stage_lds_table(void *lds, const uint32_t *__restrict__ tab,
+                                                            uint32_t k, uint32_t m, uint32_t stride_t,
+                                                            uint32_t stride_j) {
+    constexpr uint32_t E = B * B;
+    uint4 *lt = (uint4 *)lds;
+    uint32_t *l2 = (uint32_t *)(lt + k * m * E);
+    for (uint32_t i = threadIdx.x; i < k * m * E; i += blockDim.x) {
+        const uint32_t tj = i / E, pq = i - tj * E;
+        const uint32_t t = tj / m, j = tj - t * m;
+        const uint32_t *e = tab + (size_t)t * stride_t + (size_t)j * stride_j + pq * TAB_DW;
         lt[i] = make_uint4(e[0], e[1], e[2], e[3]);
+        if (T2LDS) l2[i] = e[4];
     }
     __syncthreads();
+    return {lt, l2, tab, stride_t, stride_j, m};
 }
 
 // Folds S (1 or 2) sources (dwords x0, x1) into the row accumulators, with

@@ -455,20 +455,23 @@ def test_unaligned_and_mixed_alignment_buffers(f):
                 assert np.array_equal(out[do[j]:do[j] + row_len], want), (offs, row_len, j)
 
 
-def test_many_sources_and_rows_are_chunked():
-    # k > 64 sources and m > 16 rows exceed one launch's argument block.
+@pytest.mark.parametrize("name", ["gf8d", "gf16", "gf32", "gf64"])
+def test_many_sources_and_rows_are_chunked(name):
+    # k > 64 sources and m > 16 rows exceed one launch's argument block (and,
+    # for the wider fields, one launch's table staging).
+    f = BY_NAME[name]
     row_len, k, m = 4096 + 48, 70, 20
     srcs = [noise(row_len, 0xC00 + t) for t in range(k)]
-    coeffs = rand_coeffs(Gf8D, k * m, 0xC01)
-    plan = ops.Plan.matrix(Gf8D, k, m, coeffs)
+    coeffs = rand_coeffs(f, k * m, 0xC01)
+    plan = ops.Plan.matrix(f, k, m, coeffs)
     d = dev(noise(row_len * m, 0xC02))
-    ops.dot_product_matrix_with(Gf8D, d, row_len, m, plan, [dev(s) for s in srcs])
-    want = oracle_matrix("gf8d", row_len, m, coeffs, srcs)
+    ops.dot_product_matrix_with(f, d, row_len, m, plan, [dev(s) for s in srcs])
+    want = oracle_matrix(name, row_len, m, coeffs, srcs)
     assert np.array_equal(host(d), want)
     init = noise(row_len * m, 0xC03)
     d = dev(init)
-    ops.mul_add_matrix_with(Gf8D, d, row_len, m, plan, [dev(s) for s in srcs])
-    assert np.array_equal(host(d), oracle_matrix("gf8d", row_len, m, coeffs, srcs, init))
+    ops.mul_add_matrix_with(f, d, row_len, m, plan, [dev(s) for s in srcs])
+    assert np.array_equal(host(d), oracle_matrix(name, row_len, m, coeffs, srcs, init))
 
 
 def cauchy(k, m):
