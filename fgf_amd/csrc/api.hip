@@ -45,7 +45,7 @@ int device_fail(const char *what, hipError_t e) {
     return FGF_EDEVICE;
 }
 
-size_t entry_dw(int nb) { return (size_t)nb * nb * TAB_DW; }
+size_t entry_dw(int nb) { return entry_dwords(nb); }
 
 bool field_ok(int f) { return valid_field(f); }
 
@@ -135,6 +135,7 @@ int run_matrix(int f, const uint32_t *tab, size_t stride_t, size_t stride_j, uin
                hipStream_t s) {
     MatrixJob job;
     job.nb = field_bytes(f);
+    job.fan_paar = is_fan_paar(f);
     job.srcs = srcs;
     job.k = k;
     job.rows = rows;

@@ -30,7 +30,7 @@ namespace {
 
 constexpr int BATCH_BLOCK = 256;
 
-template <int B, int M, int D>
+template <int B, bool FP, int M, int D>
 __global__ __launch_bounds__(BATCH_BLOCK) void k_batch(const BatchArgs a) {
     const uint32_t lane = threadIdx.x & 63u;
     constexpr uint64_t CHUNK = 256u * D;
@@ -47,7 +47,7 @@ __global__ __launch_bounds__(BATCH_BLOCK) void k_batch(const BatchArgs a) {
         const uint32_t mrows = min((uint32_t)M, s.m - j0);
         uint64_t tile = (uint64_t)w.chunk * a.subtiles * CHUNK;
         for (uint32_t c = 0; c < a.subtiles && tile < s.row_len; ++c, tile += CHUNK)
-            core::matrix_tile<B, M, D, true>(
+            core::matrix_tile<B, M, D, true, FP>(
                 s.srcs, s.rows + j0,
                 core::GlobalTab<B>{s.tab + (size_t)j0 * s.tab_stride_j, s.tab_stride_t,
                                    s.tab_stride_j},
@@ -67,17 +67,23 @@ __global__ __launch_bounds__(BATCH_BLOCK) void k_batch(const BatchArgs a) {
 
 using BatchFn = void (*)(BatchArgs);
 
-BatchFn pick_batch(int nb, int d, int rows) {
+template <bool FP>
+BatchFn pick_batch_fp(int nb, int d, int rows) {
     switch (nb) {
-    case 1:
-        if (rows == 4) return d == 4 ? (BatchFn)k_batch<1, 4, 4> : (BatchFn)k_batch<1, 4, 8>;
-        return d == 4 ? (BatchFn)k_batch<1, 8, 4> : (BatchFn)k_batch<1, 8, 8>;
     case 2:
-        if (rows == 4) return d == 4 ? (BatchFn)k_batch<2, 4, 4> : (BatchFn)k_batch<2, 4, 8>;
-        return d == 4 ? (BatchFn)k_batch<2, 8, 4> : (BatchFn)k_batch<2, 8, 8>;
-    case 4: return (BatchFn)k_batch<4, 4, 8>;
-    default: return (BatchFn)k_batch<8, 2, 8>;
+        if (rows == 4) return d == 4 ? (BatchFn)k_batch<2, FP, 4, 4> : (BatchFn)k_batch<2, FP, 4, 8>;
+        return d == 4 ? (BatchFn)k_batch<2, FP, 8, 4> : (BatchFn)k_batch<2, FP, 8, 8>;
+    case 4: return (BatchFn)k_batch<4, FP, 4, 8>;
+    default: return (BatchFn)k_batch<8, FP, 2, 8>;
     }
+}
+
+BatchFn pick_batch(int nb, bool fp, int d, int rows) {
+    if (nb == 1) {
+        if (rows == 4) return d == 4 ? (BatchFn)k_batch<1, false, 4, 4> : (BatchFn)k_batch<1, false, 4, 8>;
+        return d == 4 ? (BatchFn)k_batch<1, false, 8, 4> : (BatchFn)k_batch<1, false, 8, 8>;
+    }
+    return fp ? pick_batch_fp<true>(nb, d, rows) : pick_batch_fp<false>(nb, d, rows);
 }
 
 }  // namespace
@@ -90,9 +96,10 @@ int batch_default_rows(int nb) { return nb <= 2 ? 8 : nb == 4 ? 4 : 2; }
 
 int batch_min_dwords(int nb) { return nb >= 4 ? 8 : 4; }
 
-hipError_t launch_batch(int nb, int dwords, int rows, const BatchArgs &args, uint32_t blocks,
+hipError_t launch_batch(int field, int dwords, int rows, const BatchArgs &args, uint32_t blocks,
                         hipStream_t stream) {
-    hipLaunchKernelGGL(pick_batch(nb, dwords, rows), dim3(blocks), dim3(BATCH_BLOCK), 0, stream,
+    hipLaunchKernelGGL(pick_batch(field_bytes(field), is_fan_paar(field), dwords, rows), dim3(blocks),
+                       dim3(BATCH_BLOCK), 0, stream,
                        args);
     return hipGetLastError();
 }
@@ -273,7 +280,7 @@ int fgf_batch_create(fgf_field f, const fgf_stripe *stripes, size_t n, fgf_batch
     b->stripes.resize(n);
     b->host_ptrs = ptrs;
     b->host_offs = offs;
-    const size_t e_dw = (size_t)nb * nb * TAB_DW;
+    const size_t e_dw = entry_dwords(nb);
     for (size_t s = 0; s < n; ++s) {
         const fgf_stripe &st = stripes[s];
         const fgf_plan *p = st.plan;
@@ -331,7 +338,7 @@ int fgf_batch_run(fgf_batch *b, void *stream) {
     fgf::BatchArgs a{b->d_stripes, b->d_items, (uint32_t)b->nitems,
                      b->blocks * (fgf::batch_block_threads() / 64), (uint32_t)b->subtiles,
                      b->d_counter};
-    hipError_t e = fgf::launch_batch(b->nb, b->dwords, b->rows_per_item, a, b->blocks,
+    hipError_t e = fgf::launch_batch(b->field, b->dwords, b->rows_per_item, a, b->blocks,
                                      (hipStream_t)stream);
     return e == hipSuccess ? FGF_OK : bdev("batch launch", e);
 }
@@ -374,6 +381,7 @@ int fgf_batch_run_per_call(fgf_batch *b, void *stream) {
         const void *const *p = b->host_ptrs.data() + b->host_offs[s];
         fgf::MatrixJob job;
         job.nb = b->nb;
+        job.fan_paar = fgf::is_fan_paar(b->field);
         job.srcs = (const uint8_t *const *)p;
         job.k = d.k;
         job.rows = (uint8_t *const *)(p + d.k);

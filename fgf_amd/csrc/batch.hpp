@@ -40,7 +40,7 @@ bool batch_rows_allowed(int nb, int rows);
 int batch_default_rows(int nb);
 int batch_min_dwords(int nb);
 uint32_t batch_block_threads();
-hipError_t launch_batch(int nb, int dwords, int rows, const BatchArgs &args, uint32_t blocks,
+hipError_t launch_batch(int field, int dwords, int rows, const BatchArgs &args, uint32_t blocks,
                         hipStream_t stream);
 std::vector<WorkItem> build_work_items(const std::vector<StripeDesc> &stripes, int dwords,
                                        int rows_per_item, int subtiles, int order);

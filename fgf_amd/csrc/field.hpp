@@ -10,7 +10,8 @@
 //         generator 0x2d                        (src/field/fan_paar.rs:58-174)
 // Elements are packed little endian (src/field/mod.rs:9-15): byte q of an
 // element is its coordinate on the q-th K-basis vector, so multiplying by a
-// fixed c is a B x B matrix over K acting on the element's bytes.
+// fixed c is K-linear in the element's bytes and composes from byte-constant
+// multiplies (HostField::kara_constants).
 #pragma once
 
 #include <cstddef>
@@ -180,14 +181,38 @@ struct HostField {
         return r;
     }
 
-    // M[p][q] = byte p of c * e_q, where e_q has byte q equal to one: the
-    // K-matrix of multiplication by c in byte coordinates.
-    static void byte_matrix(int f, uint64_t c, uint8_t *m /* B*B row-major */) {
-        int nb = field_bytes(f);
-        for (int q = 0; q < nb; ++q) {
-            uint64_t col = mul(f, c, 1ull << (8 * q));
-            for (int p = 0; p < nb; ++p) m[p * nb + q] = (uint8_t)(col >> (8 * p));
+    // The byte constants whose products compose multiplication by c (a
+    // `bits`-wide element of field f) by Karatsuba, in the branch order
+    // (S, O1, O2) at every level, 3^log2(bits/8) bytes in all. With c = c0 +
+    // c1*u over the half field and x = a + b*u:
+    //   towers, u^2 = u + D (gf16.rs:143-153): c*x = (c0 a + D c1 b)
+    //     + (c1 a + (c0 + c1) b) u, so S = c0 on a, O1 = D c1 on b,
+    //     O2 = c0 + c1 on a + b: lo = S + O1, hi = S + O2.
+    //   Fan-Paar, X^2 = alpha X + 1 (fan_paar.rs:124-145): c*x = (c0 a + c1 b)
+    //     + (c1 a + (c0 + alpha c1) b) X, so S = c1 on a + b, O1 = c0 + c1 on
+    //     a, O2 = c0 + c1 + alpha c1 on b: lo = S + O1, hi = S + O2.
+    static void kara_constants(int f, uint64_t c, unsigned bits, uint8_t *out) {
+        if (bits == 8) {
+            out[0] = (uint8_t)c;
+            return;
         }
+        const unsigned h = bits / 2;
+        unsigned eh = 1;
+        for (unsigned w = 8; w < h; w *= 2) eh *= 3;
+        const uint64_t c0 = c & lowbits(h), c1 = c >> h;
+        uint64_t s, o1, o2;
+        if (is_fan_paar(f)) {
+            s = c1;
+            o1 = c0 ^ c1;
+            o2 = c0 ^ c1 ^ fp_alpha(c1, h);
+        } else {
+            s = c0;
+            o1 = gmul(tower_delta(bits), c1, h);
+            o2 = c0 ^ c1;
+        }
+        kara_constants(f, s, h, out);
+        kara_constants(f, o1, h, out + eh);
+        kara_constants(f, o2, h, out + 2 * eh);
     }
 };
 

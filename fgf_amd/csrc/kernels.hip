@@ -23,20 +23,23 @@ namespace fgf {
 // Host: table construction.
 // ---------------------------------------------------------------------------
 
+// One v_perm table set per byte constant of the coefficient: the constant
+// itself for byte fields, the Karatsuba constants of HostField::kara_constants
+// for the towers.
 void build_coeff_table(int field, uint64_t c, uint32_t *out) {
     const int nb = field_bytes(field);
     const int bf = byte_field(field);
-    uint8_t m[64];
-    HostField::byte_matrix(field, c, m);
-    for (int pq = 0; pq < nb * nb; ++pq) {
-        const uint8_t l = m[pq];
+    uint8_t m[27];
+    HostField::kara_constants(field, c, 8u * (unsigned)nb, m);
+    for (int i = 0; i < kterms(nb); ++i) {
+        const uint8_t l = m[i];
         uint8_t t0[8], t1[8], t2[4];
         for (int i = 0; i < 8; ++i) {
             t0[i] = HostField::bmul(bf, l, (uint8_t)i);
             t1[i] = HostField::bmul(bf, l, (uint8_t)(i << 3));
         }
         for (int i = 0; i < 4; ++i) t2[i] = HostField::bmul(bf, l, (uint8_t)(i << 6));
-        uint32_t *e = out + pq * TAB_DW;
+        uint32_t *e = out + i * TAB_DW;
         std::memcpy(&e[0], t0, 8);
         std::memcpy(&e[2], t1, 8);
         std::memcpy(&e[4], t2, 4);
@@ -63,17 +66,14 @@ struct MatrixArgs {
 
 constexpr int BLOCK = 256;
 
-// Matrix kernels for B <= 4 read their tables from LDS (see k_matrix):
-// k*M*B*B records per block, kept within 64 KiB by the source chunk.
-// GF(2^64) keeps scalar tables: its 2.5 KiB of records per source would cut
-// the chunk to 25 sources and the occupancy to two blocks per CU, which
-// measured slower than the v_mov copies they save.
-constexpr bool lds_tables(int nb) { return nb <= 4; }
-constexpr size_t LDS_TABLE_BYTES = 64 * 1024;
-// T2 dwords go to LDS too when a block's B*B*M scalars per source would crowd
-// the SGPRs (wide fields, and byte fields with more than 8 rows).
+// Matrix kernels read their tables from LDS (see k_matrix): k*M*kterms(B)
+// records per block, kept within LDS_TABLE_BYTES by the source chunk (76 KiB:
+// two blocks per CU, and a k = 32 chunk of GF(2^64) at four rows).
+constexpr size_t LDS_TABLE_BYTES = 76 * 1024;
+// T2 dwords go to LDS too when a block's kterms(B)*M scalars per source would
+// crowd the SGPRs (towers, and byte fields with more than 8 rows).
 constexpr bool t2_lds(int nb, int m) { return nb > 1 || m > 8; }
-constexpr size_t lds_record_bytes(int nb, int m) { return (size_t)nb * nb * (t2_lds(nb, m) ? 20 : 16); }
+constexpr size_t lds_record_bytes(int nb, int m) { return (size_t)kterms(nb) * (t2_lds(nb, m) ? 20 : 16); }
 
 // Block index -> (column tile, row group). Blocks are dealt to the XCDs round
 // robin, so with several row groups the blocks b, b + 8, b + 16, ... (one XCD)
@@ -101,7 +101,7 @@ __device__ __forceinline__ bool map_block(const MatrixArgs &a, uint32_t &ct, uin
 //
 // The block first stages its rows' tables in LDS (dynamic shared memory, at
 // most LDS_TABLE_BYTES), then every wave runs its tile.
-template <int B, int M, int D, bool OW>
+template <int B, bool FP, int M, int D, bool OW>
 __global__ __launch_bounds__(BLOCK) void k_matrix(const MatrixArgs a) {
     uint32_t ct, rg;
     if (!map_block(a, ct, rg)) return;
@@ -110,22 +110,16 @@ __global__ __launch_bounds__(BLOCK) void k_matrix(const MatrixArgs a) {
     const uint64_t tile = ((uint64_t)ct * (BLOCK / 64) + wave) * (256u * D);
     const uint32_t j0 = rg * M;
     const uint32_t *tab = a.tab + (size_t)j0 * a.tab_stride_j;
-    if constexpr (lds_tables(B)) {
-        extern __shared__ uint4 s_lt[];
-        const auto tb = stage_lds_table<B, t2_lds(B, M)>(s_lt, tab, a.k, M, a.tab_stride_t,
-                                                      a.tab_stride_j);
-        if (tile >= a.ncols) return;
-        matrix_tile<B, M, D>(a.src, a.dst + j0, tb, a.k, OW, a.col0, a.ncols, tile, lane);
-    } else {
-        if (tile >= a.ncols) return;
-        matrix_tile<B, M, D>(a.src, a.dst + j0, GlobalTab<B>{tab, a.tab_stride_t, a.tab_stride_j},
-                             a.k, OW, a.col0, a.ncols, tile, lane);
-    }
+    extern __shared__ uint4 s_lt[];
+    const auto tb = stage_lds_table<B, t2_lds(B, M)>(s_lt, tab, a.k, M, a.tab_stride_t,
+                                                  a.tab_stride_j);
+    if (tile >= a.ncols) return;
+    matrix_tile<B, M, D, false, FP>(a.src, a.dst + j0, tb, a.k, OW, a.col0, a.ncols, tile, lane);
 }
 
 // Generic kernel for heads, tails and unaligned buffers: one thread per group
 // of four elements, byte loads and stores, rows one at a time.
-template <int B, bool OW>
+template <int B, bool FP, bool OW>
 __global__ __launch_bounds__(BLOCK) void k_matrix_generic(const MatrixArgs a) {
     const uint64_t gid = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
     const uint64_t rel = gid * 4u * B;
@@ -148,14 +142,19 @@ __global__ __launch_bounds__(BLOCK) void k_matrix_generic(const MatrixArgs a) {
             const uint8_t *s = a.src[t] + base;
             for (uint32_t i = 0; i < nbytes; ++i) x[i >> 2] |= (uint32_t)s[i] << (8 * (i & 3));
             to_planes<B>(x, pl);
-            Sel sel[B];
+            Sel base[B], sel[kterms(B)];
 #pragma unroll
-            for (int q = 0; q < B; ++q) sel[q] = selectors(pl[q]);
+            for (int q = 0; q < B; ++q) base[q] = selectors(pl[q]);
+            kara_expand<B, FP>(base, sel);
             const uint32_t *tj = a.tab + (size_t)t * a.tab_stride_t + (size_t)j * a.tab_stride_j;
+            if constexpr (B == 1) {
+                acc[0] = cmul_acc(acc[0], tj, sel[0]);
+            } else {
+                uint32_t lk[kterms(B)];
 #pragma unroll
-            for (int p = 0; p < B; ++p)
-#pragma unroll
-                for (int q = 0; q < B; ++q) acc[p] = cmul_acc(acc[p], tj + (p * B + q) * TAB_DW, sel[q]);
+                for (int i = 0; i < kterms(B); ++i) lk[i] = cmul(tj + i * TAB_DW, sel[i]);
+                kara_accumulate<B>(acc, lk);
+            }
         }
         from_planes<B>(acc, w);
         uint8_t *o = a.dst[j] + base;
@@ -170,47 +169,75 @@ __global__ __launch_bounds__(BLOCK) void k_matrix_generic(const MatrixArgs a) {
 using MatrixFn = void (*)(MatrixArgs);
 
 // Rows per block for each element width, bounded by the accumulator budget.
-constexpr int max_rows(int nb) { return nb == 1 ? 16 : nb == 2 ? 8 : nb == 4 ? 4 : 2; }
+constexpr int max_rows(int nb) { return nb == 1 ? 16 : nb == 2 ? 8 : 4; }
+// Dynamic LDS one block may use for tables on the current device: the
+// smaller of LDS_TABLE_BYTES and the device's per-block limit.
+size_t lds_budget() {
+    static thread_local int cached_dev = -1;
+    static thread_local size_t cached = 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    if (dev != cached_dev) {
+        int lim = 0;
+        if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+            lim <= 0)
+            lim = 64 * 1024;
+        cached = std::min<size_t>(LDS_TABLE_BYTES, (size_t)lim);
+        cached_dev = dev;
+    }
+    return cached;
+}
+
 // Sources per launch: KMAX, or fewer when a full row group's LDS tables would
-// exceed LDS_TABLE_BYTES.
-constexpr size_t k_chunk(int nb) {
-    return lds_tables(nb) ? std::min<size_t>(KMAX, LDS_TABLE_BYTES / (max_rows(nb) * lds_record_bytes(nb, max_rows(nb))))
-                          : (size_t)KMAX;
+// exceed the LDS budget.
+size_t k_chunk(int nb) {
+    const size_t per_source = (size_t)max_rows(nb) * lds_record_bytes(nb, max_rows(nb));
+    return std::max<size_t>(1, std::min<size_t>(KMAX, lds_budget() / per_source));
 }
 // Byte fields: 16 bytes per lane per source (1 KiB per wave per stream), the
 // granularity at which the encode mix streams fastest; wider elements keep
 // 32 bytes so a lane holds whole 4-element groups.
 constexpr int dwords_for(int nb, int m) { return nb == 1 ? 4 : 8; }
 
-template <int B, int M>
+template <int B, bool FP, int M>
 MatrixFn pick_m(bool ow) {
     constexpr int D = dwords_for(B, M);
-    return ow ? (MatrixFn)k_matrix<B, M, D, true> : (MatrixFn)k_matrix<B, M, D, false>;
+    return ow ? (MatrixFn)k_matrix<B, FP, M, D, true> : (MatrixFn)k_matrix<B, FP, M, D, false>;
 }
 
-template <int B, int... Ms>
+template <int B, bool FP, int... Ms>
 MatrixFn pick_from(int m, bool ow, std::integer_sequence<int, Ms...>) {
     MatrixFn fn = nullptr;
-    ((m == Ms + 1 ? (fn = pick_m<B, Ms + 1>(ow), 0) : 0), ...);
+    ((m == Ms + 1 ? (fn = pick_m<B, FP, Ms + 1>(ow), 0) : 0), ...);
     return fn;
 }
 
-MatrixFn pick_vector(int nb, int m, bool ow) {
+template <bool FP>
+MatrixFn pick_vector_fp(int nb, int m, bool ow) {
     switch (nb) {
-    case 1: return pick_from<1>(m, ow, std::make_integer_sequence<int, 16>{});
-    case 2: return pick_from<2>(m, ow, std::make_integer_sequence<int, 8>{});
-    case 4: return pick_from<4>(m, ow, std::make_integer_sequence<int, 4>{});
-    default: return pick_from<8>(m, ow, std::make_integer_sequence<int, 2>{});
+    case 2: return pick_from<2, FP>(m, ow, std::make_integer_sequence<int, max_rows(2)>{});
+    case 4: return pick_from<4, FP>(m, ow, std::make_integer_sequence<int, max_rows(4)>{});
+    default: return pick_from<8, FP>(m, ow, std::make_integer_sequence<int, max_rows(8)>{});
     }
 }
 
-MatrixFn pick_generic(int nb, bool ow) {
+MatrixFn pick_vector(int nb, bool fp, int m, bool ow) {
+    if (nb == 1) return pick_from<1, false>(m, ow, std::make_integer_sequence<int, max_rows(1)>{});
+    return fp ? pick_vector_fp<true>(nb, m, ow) : pick_vector_fp<false>(nb, m, ow);
+}
+
+template <bool FP>
+MatrixFn pick_generic_fp(int nb, bool ow) {
     switch (nb) {
-    case 1: return ow ? (MatrixFn)k_matrix_generic<1, true> : (MatrixFn)k_matrix_generic<1, false>;
-    case 2: return ow ? (MatrixFn)k_matrix_generic<2, true> : (MatrixFn)k_matrix_generic<2, false>;
-    case 4: return ow ? (MatrixFn)k_matrix_generic<4, true> : (MatrixFn)k_matrix_generic<4, false>;
-    default: return ow ? (MatrixFn)k_matrix_generic<8, true> : (MatrixFn)k_matrix_generic<8, false>;
+    case 2: return ow ? (MatrixFn)k_matrix_generic<2, FP, true> : (MatrixFn)k_matrix_generic<2, FP, false>;
+    case 4: return ow ? (MatrixFn)k_matrix_generic<4, FP, true> : (MatrixFn)k_matrix_generic<4, FP, false>;
+    default: return ow ? (MatrixFn)k_matrix_generic<8, FP, true> : (MatrixFn)k_matrix_generic<8, FP, false>;
     }
+}
+
+MatrixFn pick_generic(int nb, bool fp, bool ow) {
+    if (nb == 1) return ow ? (MatrixFn)k_matrix_generic<1, false, true> : (MatrixFn)k_matrix_generic<1, false, false>;
+    return fp ? pick_generic_fp<true>(nb, ow) : pick_generic_fp<false>(nb, ow);
 }
 
 hipError_t launch_segment(const MatrixJob &job, MatrixArgs a, size_t k0, size_t kn, size_t j0,
@@ -244,14 +271,14 @@ hipError_t launch_segment(const MatrixJob &job, MatrixArgs a, size_t k0, size_t 
         const uint64_t blocks = a.ngroups == 1
                                     ? a.ncoltiles
                                     : (uint64_t)((a.ncoltiles + 7) / 8) * 8u * a.ngroups;
-        const size_t lds = lds_tables(nb) ? kn * (size_t)mt * lds_record_bytes(nb, mt) : 0;
-        hipLaunchKernelGGL(pick_vector(nb, mt, ow), dim3((uint32_t)blocks), dim3(BLOCK), lds,
+        const size_t lds = kn * (size_t)mt * lds_record_bytes(nb, mt);
+        hipLaunchKernelGGL(pick_vector(nb, job.fan_paar, mt, ow), dim3((uint32_t)blocks), dim3(BLOCK), lds,
                            job.stream, a);
     } else {
         a.ngroups = 1;
         const uint64_t groups = (ncols / nb + 3) / 4;
         a.ncoltiles = (uint32_t)groups;
-        hipLaunchKernelGGL(pick_generic(nb, ow), dim3((uint32_t)((groups + BLOCK - 1) / BLOCK)),
+        hipLaunchKernelGGL(pick_generic(nb, job.fan_paar, ow), dim3((uint32_t)((groups + BLOCK - 1) / BLOCK)),
                            dim3(BLOCK), 0, job.stream, a);
     }
     return hipGetLastError();

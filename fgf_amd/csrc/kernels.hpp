@@ -17,12 +17,21 @@ constexpr int TAB_DW = 5;
 constexpr int KMAX = 64;
 constexpr int MMAX = 16;
 
+// Byte constants per coefficient: 1 for byte fields, 3^log2(B) for the
+// towers, whose constant multiply is applied by Karatsuba composition over
+// the byte field (3 half-width multiplies per level instead of 4).
+constexpr int kterms(int nb) { return nb == 1 ? 1 : nb == 2 ? 3 : nb == 4 ? 9 : 27; }
+
+// Dwords of one coefficient's tables.
+constexpr size_t entry_dwords(int nb) { return (size_t)kterms(nb) * TAB_DW; }
+
 // One matrix operation: rows[j] (op)= sum_t c(t, j) * srcs[t] over row_len
 // bytes. The table entry of coefficient (t, j) starts at
-// tab + t * tab_stride_t + j * tab_stride_j (dwords) and holds B*B byte-constant
-// tables in row-major (p, q) order: out byte p += M[p][q] * in byte q.
+// tab + t * tab_stride_t + j * tab_stride_j (dwords) and holds the kterms(B)
+// byte-constant tables that build_coeff_table lays out.
 struct MatrixJob {
     int nb = 1;                          // field bytes B
+    bool fan_paar = false;               // Fan-Paar tower (Karatsuba branch order)
     const uint8_t *const *srcs = nullptr;  // host array of k device pointers
     size_t k = 0;
     uint8_t *const *rows = nullptr;        // host array of m device pointers
@@ -44,7 +53,7 @@ hipError_t launch_elementwise(int field, uint8_t *dst, const uint8_t *a, const u
 // dst[0..len) = 0.
 hipError_t launch_fill_zero(uint8_t *dst, size_t len, hipStream_t stream);
 
-// Host-side table builder: writes the B*B*TAB_DW dwords of coefficient c.
+// Host-side table builder: writes the entry_dwords(B) dwords of coefficient c.
 void build_coeff_table(int field, uint64_t c, uint32_t *out);
 
 // Sets the thread-local text returned by fgf_last_error().

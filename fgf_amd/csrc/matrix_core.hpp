@@ -2,13 +2,19 @@
 // (kernels.hip) and the persistent batch kernel (batch.hip).
 //
 // rows[j] (=|^=) sum_t c(t, j) * src[t] over one wave's column tile.
-// Multiplying a packed element by a fixed coefficient is a B x B matrix over
-// the byte subfield (src/kernel/tables.rs:223-237 states this for GF(2^16)).
 // Each byte constant l is applied to four bytes at a time with three
 // v_perm_b32 lookups on the 3-, 3- and 2-bit slices of every byte
 // (l*x = T0[x & 7] ^ T1[(x >> 3) & 7] ^ T2[x >> 6], by GF(2)-linearity).
 // Elements of width B are transposed in registers into B byte planes of four
-// elements each, so each plane meets one constant per (p, q). The wave keeps
+// elements each. A tower constant multiply is composed by Karatsuba over the
+// byte field: with x = a + b*u (a, b the low and high halves), c*x needs
+// three half-width constant multiplies S, O1, O2 and gives
+// lo = S ^ O1, hi = S ^ O2 (build_coeff_table derives the three half-width
+// constants of each field family). Applied recursively down to bytes that
+// is 3^log2(B) byte constants per element instead of the B*B entries of the
+// element's byte matrix (src/kernel/tables.rs:223-237), on planes a, b and
+// a ^ b whose v_perm selectors are XORs of the base planes' selectors (the
+// selector masks and shifts commute with XOR). The wave keeps
 // the M destination rows of its tile in VGPRs while it folds in all k
 // sources: every source byte crosses HBM once and every destination byte is
 // written once (the register blocking of the reference's x86 matrix_impl,
@@ -124,6 +130,77 @@ __device__ __forceinline__ void cmul_fold(Fold &f, const uint32_t *__restrict__ 
     f.add(vperm(e[4], e[4], s.s2));
 }
 
+// l * v for the byte constant whose tables start at e.
+__device__ __forceinline__ uint32_t cmul(const uint32_t *__restrict__ e, const Sel &s) {
+    return xor3(vperm(e[1], e[0], s.s0), vperm(e[3], e[2], s.s1), vperm(e[4], e[4], s.s2));
+}
+
+__device__ __forceinline__ Sel sel_xor(const Sel &a, const Sel &b) {
+    return {a.s0 ^ b.s0, a.s1 ^ b.s1, a.s2 ^ b.s2};
+}
+
+// Karatsuba planes of one group of four elements: from the selectors of its
+// B byte planes, the selectors of the kterms(B) planes the byte constants
+// act on, in the table's branch order. Each level splits the element into
+// halves a (low bytes) and b (high bytes); the branches are
+//   GF(2^2^n) towers (u^2 = u + DELTA):  S on a,      O1 on b, O2 on a ^ b
+//   Fan-Paar (X^2 = alpha X + 1):        S on a ^ b,  O1 on a, O2 on b
+template <int B, bool FP>
+__device__ __forceinline__ void kara_expand(const Sel *base, Sel *ex) {
+    if constexpr (B == 1) {
+        ex[0] = base[0];
+    } else {
+        constexpr int H = B / 2, EH = kterms(H);
+        Sel mid[H];
+#pragma unroll
+        for (int q = 0; q < H; ++q) mid[q] = sel_xor(base[q], base[H + q]);
+        if constexpr (FP) {
+            kara_expand<H, FP>(mid, ex);
+            kara_expand<H, FP>(base, ex + EH);
+            kara_expand<H, FP>(base + H, ex + 2 * EH);
+        } else {
+            kara_expand<H, FP>(base, ex);
+            kara_expand<H, FP>(base + H, ex + EH);
+            kara_expand<H, FP>(mid, ex + 2 * EH);
+        }
+    }
+}
+
+// The B product planes from the kterms(B) byte-constant products lk, branch
+// order (S, O1, O2) at every level: lo = S ^ O1, hi = S ^ O2.
+template <int B>
+__device__ __forceinline__ void kara_combine(const uint32_t *lk, uint32_t *out) {
+    if constexpr (B == 1) {
+        out[0] = lk[0];
+    } else {
+        constexpr int H = B / 2, EH = kterms(H);
+        uint32_t s[H], o1[H], o2[H];
+        kara_combine<H>(lk, s);
+        kara_combine<H>(lk + EH, o1);
+        kara_combine<H>(lk + 2 * EH, o2);
+#pragma unroll
+        for (int q = 0; q < H; ++q) {
+            out[q] = s[q] ^ o1[q];
+            out[H + q] = s[q] ^ o2[q];
+        }
+    }
+}
+
+// acc (B planes) ^= the product, with the top level folded into v_bitop3.
+template <int B>
+__device__ __forceinline__ void kara_accumulate(uint32_t *acc, const uint32_t *lk) {
+    constexpr int H = B / 2, EH = kterms(H);
+    uint32_t s[H], o1[H], o2[H];
+    kara_combine<H>(lk, s);
+    kara_combine<H>(lk + EH, o1);
+    kara_combine<H>(lk + 2 * EH, o2);
+#pragma unroll
+    for (int q = 0; q < H; ++q) {
+        acc[q] = xor3(acc[q], s[q], o1[q]);
+        acc[H + q] = xor3(acc[H + q], s[q], o2[q]);
+    }
+}
+
 // Per-lane geometry of one wave's tile: D dwords per lane per source in
 // 16-byte pieces at base + lo[i], lo[i] = i*1024 + lane*16, so each load
 // instruction covers 1 KiB contiguous. `base` (= col0 + tile) is uniform;
@@ -162,8 +239,8 @@ __device__ __forceinline__ void load_source(uint32_t (&x)[D], SrcPtrs src, uint3
 }
 
 // Coefficient-table views. An entry is the five dwords of one byte constant
-// (T0 lo/hi, T1 lo/hi, T2); the entry of source t + s, row j, plane (p, q)
-// is at tt + s*stride_t + j*stride_j + (p*B + q)*TAB_DW in the plan's table.
+// (T0 lo/hi, T1 lo/hi, T2); byte constant i of source t + s, row j is at
+// tt + s*stride_t + j*stride_j + i*TAB_DW in the plan's table.
 //
 // GlobalTab reads entries with wave-uniform addresses (scalar loads).
 // v_perm_b32 takes one scalar operand, so each T0/T1 pair then costs a v_mov
@@ -175,17 +252,20 @@ struct GlobalTab {
     __device__ __forceinline__ GlobalTab at(uint32_t dt) const {
         return {tt + __builtin_amdgcn_readfirstlane(dt * stride_t), stride_t, stride_j};
     }
-    __device__ __forceinline__ void fold(Fold &f, int s, int j, int p, int q, const Sel &sel) const {
-        cmul_fold(f, tt + s * stride_t + j * stride_j + (p * B + q) * TAB_DW, sel);
+    __device__ __forceinline__ void fold(Fold &f, int s, int j, int i, const Sel &sel) const {
+        cmul_fold(f, tt + s * stride_t + j * stride_j + i * TAB_DW, sel);
+    }
+    __device__ __forceinline__ uint32_t lookup(int s, int j, int i, const Sel &sel) const {
+        return cmul(tt + s * stride_t + j * stride_j + i * TAB_DW, sel);
     }
 };
 
 // LdsTab: the tables of the block's rows are staged in LDS, one 16-byte
-// record {T0 lo, T0 hi, T1 lo, T1 hi} per (source, row, plane pair), read
+// record {T0 lo, T0 hi, T1 lo, T1 hi} per (source, row, byte constant), read
 // back with broadcast ds_read_b128 straight into the VGPRs v_perm needs. T2
 // (one dword, both v_perm halves) stays a scalar operand for byte fields
-// (T2LDS false); wider fields, whose B*B entries per (source, row) would
-// crowd the SGPRs, read it from a second LDS array of dwords after the
+// (T2LDS false); wider fields, whose kterms(B) entries per (source, row)
+// would crowd the SGPRs, read it from a second LDS array of dwords after the
 // records.
 template <int B, bool T2LDS>
 struct LdsTab {
@@ -193,34 +273,41 @@ struct LdsTab {
     const uint32_t *l2;                // LDS T2 dwords of source t, row 0 (T2LDS)
     const uint32_t *__restrict__ tt;   // plan table of source t, row 0 (T2 when !T2LDS)
     uint32_t stride_t, stride_j, m;    // m: rows per source in the LDS records
+    static constexpr int E = kterms(B);
     __device__ __forceinline__ LdsTab at(uint32_t dt) const {
-        return {lt + dt * m * (B * B), l2 + dt * m * (B * B),
+        return {lt + dt * m * E, l2 + dt * m * E,
                 tt + __builtin_amdgcn_readfirstlane(dt * stride_t), stride_t, stride_j, m};
     }
-    __device__ __forceinline__ void fold(Fold &f, int s, int j, int p, int q, const Sel &sel) const {
-        const int r = (s * m + j) * (B * B) + p * B + q;
+    __device__ __forceinline__ void fold(Fold &f, int s, int j, int i, const Sel &sel) const {
+        const int r = (s * m + j) * E + i;
         const uint4 e = lt[r];
-        const uint32_t t2 = T2LDS ? l2[r] : tt[s * stride_t + j * stride_j + (p * B + q) * TAB_DW + 4];
+        const uint32_t t2 = T2LDS ? l2[r] : tt[s * stride_t + j * stride_j + i * TAB_DW + 4];
         f.add(vperm(e.y, e.x, sel.s0));
         f.add(vperm(e.w, e.z, sel.s1));
         f.add(vperm(t2, t2, sel.s2));
+    }
+    __device__ __forceinline__ uint32_t lookup(int s, int j, int i, const Sel &sel) const {
+        const int r = (s * m + j) * E + i;
+        const uint4 e = lt[r];
+        const uint32_t t2 = T2LDS ? l2[r] : tt[s * stride_t + j * stride_j + i * TAB_DW + 4];
+        return xor3(vperm(e.y, e.x, sel.s0), vperm(e.w, e.z, sel.s1), vperm(t2, t2, sel.s2));
     }
 };
 
 // LDS bytes that stage_lds_table uses for k sources and m rows.
 template <int B, bool T2LDS>
 constexpr size_t lds_table_bytes(size_t k, size_t m) {
-    return k * m * B * B * (T2LDS ? 20u : 16u);
+    return k * m * kterms(B) * (T2LDS ? 20u : 16u);
 }
 
 // Stages the tables of rows [0, m) of sources [0, k) into LDS: record
-// (t*m + j)*B*B + p*B + q, then (T2LDS) the T2 dwords in the same order.
+// (t*m + j)*kterms(B) + i, then (T2LDS) the T2 dwords in the same order.
 // Every thread of the block must call it (it ends with a barrier).
 template <int B, bool T2LDS>
 __device__ __forceinline__ LdsTab<B, T2LDS> stage_lds_table(void *lds, const uint32_t *__restrict__ tab,
                                                             uint32_t k, uint32_t m, uint32_t stride_t,
                                                             uint32_t stride_j) {
-    constexpr uint32_t E = B * B;
+    constexpr uint32_t E = kterms(B);
     uint4 *lt = (uint4 *)lds;
     uint32_t *l2 = (uint32_t *)(lt + k * m * E);
     for (uint32_t i = threadIdx.x; i < k * m * E; i += blockDim.x) {
@@ -235,90 +322,101 @@ __device__ __forceinline__ LdsTab<B, T2LDS> stage_lds_table(void *lds, const uin
 }
 
 // Folds S (1 or 2) sources (dwords x0, x1) into the row accumulators, with
-// the tables of sources t and t + 1 seen through `tb`. Each (row, plane)
-// folds all of its 3*B*S lookups with v_bitop3. Two loop orders:
+// the tables of sources t and t + 1 seen through `tb`.
+//
+// Byte fields: each row folds all of its 3*S lookups with v_bitop3. Two loop
+// orders:
 //  - dword-outer: one dword's selectors live, every table of the step held
-//    in SGPRs (S*M*5 dwords) -- byte fields with few rows;
+//    in SGPRs (S*M*5 dwords) -- few rows;
 //  - row-outer: all selectors of the step live, one row's tables at a time.
+// Towers (S = 1): the selectors of every group's Karatsuba planes live, then
+// per row the kterms(B) byte-constant products of each group go through the
+// Karatsuba network into the row's B accumulator planes.
 // With GUARD, only rows j < mrows are folded (row-outer, so each row's tables
 // are still loaded once): one body serves every row count up to M.
 template <int B, int M>
 constexpr bool dword_outer() { return B == 1 && M <= 6; }
 
-template <int B, int M, int D, int S, bool GUARD, typename Tab>
+template <int B, int M, int D, int S, bool GUARD, bool FP, typename Tab>
 __device__ __forceinline__ void consume(uint32_t (&acc)[M][D], const uint32_t *x0,
                                         const uint32_t *x1, const Tab &tb, uint32_t mrows) {
     constexpr int G = D / B;
-    const uint32_t *xs[2] = {x0, x1};
-    if constexpr (!GUARD && dword_outer<B, M>()) {
+    if constexpr (B == 1) {
+        const uint32_t *xs[2] = {x0, x1};
+        if constexpr (!GUARD && dword_outer<B, M>()) {
 #pragma unroll
-        for (int g = 0; g < G; ++g) {
-            Sel sel[S][B];
+            for (int g = 0; g < G; ++g) {
+                Sel sel[S];
 #pragma unroll
-            for (int s = 0; s < S; ++s) {
-                uint32_t pl[B];
-                to_planes<B>(&xs[s][g * B], pl);
+                for (int s = 0; s < S; ++s) sel[s] = selectors(xs[s][g]);
 #pragma unroll
-                for (int q = 0; q < B; ++q) sel[s][q] = selectors(pl[q]);
+                for (int j = 0; j < M; ++j) {
+                    Fold f{acc[j][g], 0u, false};
+#pragma unroll
+                    for (int s = 0; s < S; ++s) tb.fold(f, s, j, 0, sel[s]);
+                    acc[j][g] = f.done();
+                }
             }
+        } else {
+            Sel sel[S][D];
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+#pragma unroll
+                for (int g = 0; g < G; ++g) sel[s][g] = selectors(xs[s][g]);
 #pragma unroll
             for (int j = 0; j < M; ++j) {
+                if (GUARD && j >= (int)mrows) continue;
 #pragma unroll
-                for (int p = 0; p < B; ++p) {
-                    Fold f{acc[j][g * B + p], 0u, false};
+                for (int g = 0; g < G; ++g) {
+                    Fold f{acc[j][g], 0u, false};
 #pragma unroll
-                    for (int s = 0; s < S; ++s)
-#pragma unroll
-                        for (int q = 0; q < B; ++q) tb.fold(f, s, j, p, q, sel[s][q]);
-                    acc[j][g * B + p] = f.done();
+                    for (int s = 0; s < S; ++s) tb.fold(f, s, j, 0, sel[s][g]);
+                    acc[j][g] = f.done();
                 }
             }
         }
     } else {
-        Sel sel[S][D];
+        static_assert(S == 1, "towers consume one source at a time");
+        constexpr int E = kterms(B);
+        Sel sel[G][E];
 #pragma unroll
-        for (int s = 0; s < S; ++s)
+        for (int g = 0; g < G; ++g) {
+            uint32_t pl[B];
+            to_planes<B>(&x0[g * B], pl);
+            Sel base[B];
 #pragma unroll
-            for (int g = 0; g < G; ++g) {
-                uint32_t pl[B];
-                to_planes<B>(&xs[s][g * B], pl);
-#pragma unroll
-                for (int q = 0; q < B; ++q) sel[s][g * B + q] = selectors(pl[q]);
-            }
+            for (int q = 0; q < B; ++q) base[q] = selectors(pl[q]);
+            kara_expand<B, FP>(base, sel[g]);
+        }
 #pragma unroll
         for (int j = 0; j < M; ++j) {
             if (GUARD && j >= (int)mrows) continue;
+            uint32_t lk[G][E];
 #pragma unroll
-            for (int p = 0; p < B; ++p) {
+            for (int i = 0; i < E; ++i)
 #pragma unroll
-                for (int g = 0; g < G; ++g) {
-                    Fold f{acc[j][g * B + p], 0u, false};
+                for (int g = 0; g < G; ++g) lk[g][i] = tb.lookup(0, j, i, sel[g][i]);
 #pragma unroll
-                    for (int s = 0; s < S; ++s)
-#pragma unroll
-                        for (int q = 0; q < B; ++q) tb.fold(f, s, j, p, q, sel[s][g * B + q]);
-                    acc[j][g * B + p] = f.done();
-                }
-            }
+            for (int g = 0; g < G; ++g) kara_accumulate<B>(&acc[j][g * B], lk[g]);
         }
     }
 }
 
 // Folds the sources t (and t + 1 when present) held in x.
-template <int B, int M, int D, bool GUARD, typename Tab>
+template <int B, int M, int D, bool GUARD, bool FP, typename Tab>
 __device__ __forceinline__ void consume_step(uint32_t (&acc)[M][D], const uint32_t (&x)[2][D],
                                              const Tab &tab, uint32_t t, uint32_t k,
                                              uint32_t mrows) {
     const Tab tb = tab.at(t);
     if (t + 1 < k) {
         if constexpr (B == 1) {
-            consume<B, M, D, 2, GUARD>(acc, x[0], x[1], tb, mrows);
+            consume<B, M, D, 2, GUARD, FP>(acc, x[0], x[1], tb, mrows);
         } else {
-            consume<B, M, D, 1, GUARD>(acc, x[0], x[0], tb, mrows);
-            consume<B, M, D, 1, GUARD>(acc, x[1], x[1], tab.at(t + 1), mrows);
+            consume<B, M, D, 1, GUARD, FP>(acc, x[0], x[0], tb, mrows);
+            consume<B, M, D, 1, GUARD, FP>(acc, x[1], x[1], tab.at(t + 1), mrows);
         }
     } else {
-        consume<B, M, D, 1, GUARD>(acc, x[0], x[0], tb, mrows);
+        consume<B, M, D, 1, GUARD, FP>(acc, x[0], x[0], tb, mrows);
     }
 }
 
@@ -331,8 +429,8 @@ __device__ __forceinline__ void consume_step(uint32_t (&acc)[M][D], const uint32
 // (seed from the destination). Sources are consumed in pairs from two
 // ping-pong register buffers: the next pair's loads are in flight while the
 // current pair is multiplied.
-template <int B, int M, int D, bool GUARD = false, typename SrcPtrs, typename DstPtrs,
-          typename Tab>
+template <int B, int M, int D, bool GUARD = false, bool FP = false, typename SrcPtrs,
+          typename DstPtrs, typename Tab>
 __device__ __forceinline__ void matrix_tile(SrcPtrs src, DstPtrs dst, const Tab &tab, uint32_t k,
                                             bool ow, uint64_t col0, uint64_t ncols,
                                             uint64_t tile, uint32_t lane, uint32_t mrows = M) {
@@ -368,12 +466,12 @@ __device__ __forceinline__ void matrix_tile(SrcPtrs src, DstPtrs dst, const Tab 
     for (uint32_t t = 0;;) {
         if (t + 2 < k) load_source<D>(xb[0], src, t + 2, base, tl);
         if (t + 3 < k) load_source<D>(xb[1], src, t + 3, base, tl);
-        consume_step<B, M, D, GUARD>(acc, xa, tab, t, k, mrows);
+        consume_step<B, M, D, GUARD, FP>(acc, xa, tab, t, k, mrows);
         t += 2;
         if (t >= k) break;
         if (t + 2 < k) load_source<D>(xa[0], src, t + 2, base, tl);
         if (t + 3 < k) load_source<D>(xa[1], src, t + 3, base, tl);
-        consume_step<B, M, D, GUARD>(acc, xb, tab, t, k, mrows);
+        consume_step<B, M, D, GUARD, FP>(acc, xb, tab, t, k, mrows);
         t += 2;
         if (t >= k) break;
     }

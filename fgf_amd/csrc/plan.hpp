@@ -10,6 +10,6 @@ struct fgf_plan {
     int nb = 1;
     size_t rows = 0, cols = 0;
     std::vector<uint64_t> values;
-    uint32_t *d_tab = nullptr;  // entry i at i * nb * nb * TAB_DW dwords
+    uint32_t *d_tab = nullptr;  // entry i at i * entry_dwords(nb) dwords
     int device = 0;
 };

@@ -285,7 +285,7 @@ int launch_slot(fgf_server *s, Slot &sl) {
     if (e == hipSuccess) e = grow(sl.d_stripes, sl.h_stripes, sl.cap_stripes, stripes.size());
     if (e != hipSuccess) return sdev("slot buffers", e);
     size_t p = 0;
-    const size_t edw = (size_t)nb * nb * TAB_DW;
+    const size_t edw = entry_dwords(nb);
     for (size_t i = 0; i < sl.reqs.size(); ++i) {
         const Req &q = sl.reqs[i];
         const fgf_plan *plan = sl.plans[i]->plan;
@@ -327,7 +327,7 @@ int launch_slot(fgf_server *s, Slot &sl) {
     const uint32_t blocks = (uint32_t)((waves + wpb - 1) / wpb);
     BatchArgs a{sl.d_stripes, sl.d_items, (uint32_t)items.size(), blocks * wpb,
                 (uint32_t)s->cfg.subtiles, sl.d_counter};
-    if ((e = launch_batch(nb, s->cfg.dwords, s->cfg.rows_per_item, a, blocks, sl.stream)) ||
+    if ((e = launch_batch(f, s->cfg.dwords, s->cfg.rows_per_item, a, blocks, sl.stream)) ||
         (e = hipEventRecord(sl.done, sl.stream)))
         return sdev("batch launch", e);
     sl.busy = true;

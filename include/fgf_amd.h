@@ -191,11 +191,15 @@ int fgf_mul_elementwise(fgf_field field, uint8_t *dst, size_t dst_len, const uin
                         size_t a_len, const uint8_t *b, size_t b_len, void *stream);
 
 /* Diagnostic: the device table of one coefficient as the kernels read it,
- * B*B entries of 5 dwords {T0 lo, T0 hi, T1 lo, T1 hi, T2}, entry (p, q) for
- * output byte p from input byte q, with l*x = T0[x & 7] ^ T1[(x >> 3) & 7] ^
- * T2[x >> 6] per byte. No counterpart in the reference (its prepared forms
- * are private, src/kernel/tables.rs); it lets a CPU model check the table
- * math. `out` holds fgf_field_bytes(field)^2 * 5 dwords. */
+ * E entries of 5 dwords {T0 lo, T0 hi, T1 lo, T1 hi, T2}, one per byte
+ * constant l, with l*x = T0[x & 7] ^ T1[(x >> 3) & 7] ^ T2[x >> 6] per byte.
+ * E = 1 for byte fields; for the towers E = 3^log2(B) constants of the
+ * Karatsuba composition of the multiply (recursive branch order S, O1, O2:
+ * on the low half, high half and their XOR for GF(2^16/32/64), on the XOR,
+ * low and high half for Fan-Paar; product lo = S ^ O1, hi = S ^ O2). No
+ * counterpart in the reference (its prepared forms are private,
+ * src/kernel/tables.rs); it lets a CPU model check the table math. `out`
+ * holds E * 5 dwords (at most 135). */
 int fgf_coeff_table(fgf_field field, uint64_t coeff, uint32_t *out);
 
 /* ------------------------------------------------------------------------

@@ -73,11 +73,46 @@ def vperm(hi, lo, sel):
     return out.reshape(-1).view(np.uint32)
 
 
+def kterms(nb):
+    """Byte constants per coefficient: 1, or 3^log2(B) for the towers."""
+    return {1: 1, 2: 3, 4: 9, 8: 27}[nb]
+
+
 def coeff_table(f, c):
-    n = f.BYTES * f.BYTES * 5
+    n = kterms(f.BYTES) * 5
     out = (ctypes.c_uint32 * n)()
     assert lib().fgf_coeff_table(f.id, c, out) == 0
-    return np.array(out, dtype=np.uint32).reshape(f.BYTES, f.BYTES, 5)
+    return np.array(out, dtype=np.uint32).reshape(kterms(f.BYTES), 5)
+
+
+def selectors(v):
+    return (v & np.uint32(0x07070707), (v >> np.uint32(3)) & np.uint32(0x07070707),
+            (v >> np.uint32(6)) & np.uint32(0x03030303))
+
+
+def kara_expand(base, fan_paar):
+    """Selectors of the Karatsuba planes from those of the B byte planes, in
+    the table's branch order: towers (a, b, a ^ b), Fan-Paar (a ^ b, a, b),
+    where a and b are the low and high halves. The selectors of a ^ b are the
+    XOR of those of a and b."""
+    B = len(base)
+    if B == 1:
+        return list(base)
+    H = B // 2
+    lo, hi = base[:H], base[H:]
+    mid = [tuple(x ^ y for x, y in zip(a, b)) for a, b in zip(lo, hi)]
+    parts = (mid, lo, hi) if fan_paar else (lo, hi, mid)
+    return [s for part in parts for s in kara_expand(part, fan_paar)]
+
+
+def kara_combine(lk):
+    """Product planes from the byte-constant products: lo = S ^ O1,
+    hi = S ^ O2 at every level."""
+    if len(lk) == 1:
+        return list(lk)
+    e = len(lk) // 3
+    s, o1, o2 = kara_combine(lk[:e]), kara_combine(lk[e:2 * e]), kara_combine(lk[2 * e:])
+    return [a ^ b for a, b in zip(s, o1)] + [a ^ b for a, b in zip(s, o2)]
 
 
 def model_mul(f, c, data: np.ndarray) -> np.ndarray:
@@ -85,22 +120,16 @@ def model_mul(f, c, data: np.ndarray) -> np.ndarray:
     B = f.BYTES
     tab = coeff_table(f, c)
     x = data.reshape(-1, B)                        # elements
-    planes = [x[:, q].copy() for q in range(B)]    # byte q of every element
-    out = []
-    for p in range(B):
-        acc = np.zeros(len(x) // 4 * 4 // 4, dtype=np.uint32) if False else None
-        acc = np.zeros(x.shape[0] // 4, dtype=np.uint32)
-        for q in range(B):
-            v = planes[q].view(np.uint32)          # four elements per dword
-            s0 = v & np.uint32(0x07070707)
-            s1 = (v >> np.uint32(3)) & np.uint32(0x07070707)
-            s2 = (v >> np.uint32(6)) & np.uint32(0x03030303)
-            e = tab[p, q]
-            n = len(v)
-            acc ^= vperm(np.full(n, e[1], np.uint32), np.full(n, e[0], np.uint32), s0)
-            acc ^= vperm(np.full(n, e[3], np.uint32), np.full(n, e[2], np.uint32), s1)
-            acc ^= vperm(np.full(n, e[4], np.uint32), np.full(n, e[4], np.uint32), s2)
-        out.append(acc.view(np.uint8))
+    base = [selectors(x[:, q].copy().view(np.uint32)) for q in range(B)]  # four elements per dword
+    sel = kara_expand(base, f.id >= 5)
+    n = x.shape[0] // 4
+    lk = []
+    for (s0, s1, s2), e in zip(sel, tab):
+        v = vperm(np.full(n, e[1], np.uint32), np.full(n, e[0], np.uint32), s0)
+        v ^= vperm(np.full(n, e[3], np.uint32), np.full(n, e[2], np.uint32), s1)
+        v ^= vperm(np.full(n, e[4], np.uint32), np.full(n, e[4], np.uint32), s2)
+        lk.append(v)
+    out = [p.view(np.uint8) for p in kara_combine(lk)]
     return np.stack(out, 1).reshape(-1)
 
 
