@@ -18,6 +18,7 @@
 #include <new>
 #include <numeric>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "fgf_amd.h"
@@ -29,6 +30,27 @@ namespace fgf {
 namespace {
 
 constexpr int BATCH_BLOCK = 256;
+
+// The wave tiles of one item with exactly MC rows: a compile-time row count
+// leaves the tile body without per-row guards, so every table load of a
+// source step can be scheduled ahead of the lookups that use it.
+template <int B, bool FP, int D, int MC>
+__device__ __forceinline__ void item_tiles(const StripeDesc &s, uint32_t j0, uint64_t tile,
+                                           uint32_t subtiles, uint32_t lane) {
+    constexpr uint64_t CHUNK = 256u * D;
+    const core::GlobalTab<B> tab{s.tab + (size_t)j0 * s.tab_stride_j, s.tab_stride_t, s.tab_stride_j};
+    for (uint32_t c = 0; c < subtiles && tile < s.row_len; ++c, tile += CHUNK)
+        core::matrix_tile<B, MC, D, false, FP>(s.srcs, s.rows + j0, tab, s.k, s.overwrite != 0, 0,
+                                               s.row_len, tile, lane);
+}
+
+template <int B, bool FP, int D, int... Ms>
+__device__ __forceinline__ void item_by_rows(uint32_t mrows, const StripeDesc &s, uint32_t j0,
+                                             uint64_t tile, uint32_t subtiles, uint32_t lane,
+                                             std::integer_sequence<int, Ms...>) {
+    ((mrows == (uint32_t)(Ms + 1) ? (item_tiles<B, FP, D, Ms + 1>(s, j0, tile, subtiles, lane), 0) : 0),
+     ...);
+}
 
 template <int B, bool FP, int M, int D>
 __global__ __launch_bounds__(BATCH_BLOCK) void k_batch(const BatchArgs a) {
@@ -44,14 +66,9 @@ __global__ __launch_bounds__(BATCH_BLOCK) void k_batch(const BatchArgs a) {
         const WorkItem w = a.items[it];
         const StripeDesc s = a.stripes[w.stripe];
         const uint32_t j0 = w.rowgroup * M;
-        const uint32_t mrows = min((uint32_t)M, s.m - j0);
-        uint64_t tile = (uint64_t)w.chunk * a.subtiles * CHUNK;
-        for (uint32_t c = 0; c < a.subtiles && tile < s.row_len; ++c, tile += CHUNK)
-            core::matrix_tile<B, M, D, true, FP>(
-                s.srcs, s.rows + j0,
-                core::GlobalTab<B>{s.tab + (size_t)j0 * s.tab_stride_j, s.tab_stride_t,
-                                   s.tab_stride_j},
-                s.k, s.overwrite != 0, 0, s.row_len, tile, lane, mrows);
+        const uint32_t mrows = __builtin_amdgcn_readfirstlane(min((uint32_t)M, s.m - j0));
+        const uint64_t tile = (uint64_t)w.chunk * a.subtiles * CHUNK;
+        item_by_rows<B, FP, D>(mrows, s, j0, tile, a.subtiles, lane, std::make_integer_sequence<int, M>{});
         it = __builtin_amdgcn_readfirstlane(claim);
     }
     // Retire: the last wave out resets the queue for the next launch.
