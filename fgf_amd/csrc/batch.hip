@@ -31,17 +31,25 @@ namespace {
 
 constexpr int BATCH_BLOCK = 256;
 
-// The wave tiles of one item with exactly MC rows: a compile-time row count
-// leaves the tile body without per-row guards, so every table load of a
-// source step can be scheduled ahead of the lookups that use it.
+// Source slots of the per-wave load ring (matrix_run): three loads in flight
+// while one is folded in, fewer for the wide towers whose Karatsuba
+// selectors already hold most of the registers.
+constexpr int ring_slots(int nb) { return nb <= 2 ? 4 : nb == 4 ? 3 : 2; }
+
+// The wave tiles of one item with exactly MC rows, as one pipelined run: a
+// compile-time row count leaves the body without per-row guards, so every
+// table load of a source step can be scheduled ahead of the lookups that
+// use it.
 template <int B, bool FP, int D, int MC>
 __device__ __forceinline__ void item_tiles(const StripeDesc &s, uint32_t j0, uint64_t tile,
                                            uint32_t subtiles, uint32_t lane) {
     constexpr uint64_t CHUNK = 256u * D;
     const core::GlobalTab<B> tab{s.tab + (size_t)j0 * s.tab_stride_j, s.tab_stride_t, s.tab_stride_j};
-    for (uint32_t c = 0; c < subtiles && tile < s.row_len; ++c, tile += CHUNK)
-        core::matrix_tile<B, MC, D, false, FP>(s.srcs, s.rows + j0, tab, s.k, s.overwrite != 0, 0,
-                                               s.row_len, tile, lane);
+    const uint64_t left = (s.row_len - tile + CHUNK - 1) / CHUNK;
+    const uint32_t nsub = (uint32_t)min<uint64_t>(subtiles, left);
+    core::matrix_run<B, MC, D, FP, ring_slots(B)>(core::ConstPtrs<const uint8_t>{s.srcs},
+                                                  core::ConstPtrs<uint8_t>{s.rows + j0}, tab, s.k,
+                                                  s.overwrite != 0, s.row_len, tile, nsub, lane);
 }
 
 template <int B, bool FP, int D, int... Ms>
@@ -63,8 +71,8 @@ __global__ __launch_bounds__(BATCH_BLOCK) void k_batch(const BatchArgs a) {
     uint32_t it = __builtin_amdgcn_readfirstlane(claim);
     while (it < a.nitems) {
         if (lane == 0) claim = atomicAdd(&a.counter[0], 1u);
-        const WorkItem w = a.items[it];
-        const StripeDesc s = a.stripes[w.stripe];
+        const WorkItem w = core::ld_const(a.items + it);
+        const StripeDesc s = core::ld_const(a.stripes + w.stripe);
         const uint32_t j0 = w.rowgroup * M;
         const uint32_t mrows = __builtin_amdgcn_readfirstlane(min((uint32_t)M, s.m - j0));
         const uint64_t tile = (uint64_t)w.chunk * a.subtiles * CHUNK;

@@ -39,6 +39,32 @@ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
     return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
 }
 
+// Reads of memory that no kernel writes while it runs (descriptor tables,
+// pointer arrays, coefficient tables) through the constant address space:
+// at wave-uniform addresses they become scalar loads, which are counted
+// apart from the vector loads of the streams, so waiting for a table never
+// drains the stream loads in flight.
+template <typename T>
+__device__ __forceinline__ T ld_const(const T *p) {
+    static_assert(sizeof(T) % 4 == 0, "dword-sized records only");
+    typedef const __attribute__((address_space(4))) uint32_t *cu32;
+    const cu32 q = (cu32)p;
+    uint32_t w[sizeof(T) / 4];
+#pragma unroll
+    for (unsigned i = 0; i < sizeof(T) / 4; ++i) w[i] = q[i];
+    T v;
+    __builtin_memcpy(&v, w, sizeof(T));
+    return v;
+}
+
+// A device array of stream pointers read with ld_const.
+template <typename T>
+struct ConstPtrs {
+    T *const *p;
+    __device__ __forceinline__ T *operator[](uint32_t i) const { return ld_const(p + i); }
+    __device__ __forceinline__ ConstPtrs operator+(uint32_t i) const { return {p + i}; }
+};
+
 // 4 x 4 byte transpose of four dwords.
 __device__ __forceinline__ void transpose4(uint32_t &a, uint32_t &b, uint32_t &c, uint32_t &d) {
     const uint32_t t0 = vperm(b, a, 0x05010400u), t1 = vperm(b, a, 0x07030602u);
@@ -98,10 +124,21 @@ __device__ __forceinline__ Sel selectors(uint32_t v) {
     return {v & 0x07070707u, (v >> 3) & 0x07070707u, (v >> 6) & 0x03030303u};
 }
 
+// The five table dwords of one byte constant, read with wave-uniform
+// addresses (scalar loads).
+struct TabEntry {
+    uint32_t t0lo, t0hi, t1lo, t1hi, t2;
+};
+
+__device__ __forceinline__ TabEntry tab_entry(const uint32_t *e) {
+    return {ld_const(e), ld_const(e + 1), ld_const(e + 2), ld_const(e + 3), ld_const(e + 4)};
+}
+
 // acc ^ l * v for the byte constant whose tables start at e.
 __device__ __forceinline__ uint32_t cmul_acc(uint32_t acc, const uint32_t *__restrict__ e,
                                              const Sel &s) {
-    return xor3(acc, vperm(e[1], e[0], s.s0), vperm(e[3], e[2], s.s1)) ^ vperm(e[4], e[4], s.s2);
+    const TabEntry t = tab_entry(e);
+    return xor3(acc, vperm(t.t0hi, t.t0lo, s.s0), vperm(t.t1hi, t.t1lo, s.s1)) ^ vperm(t.t2, t.t2, s.s2);
 }
 
 // XOR-folds a stream of lookup results into an accumulator two at a time
@@ -125,14 +162,16 @@ struct Fold {
 
 // Adds the three lookups of l * v (tables at e) to a fold.
 __device__ __forceinline__ void cmul_fold(Fold &f, const uint32_t *__restrict__ e, const Sel &s) {
-    f.add(vperm(e[1], e[0], s.s0));
-    f.add(vperm(e[3], e[2], s.s1));
-    f.add(vperm(e[4], e[4], s.s2));
+    const TabEntry t = tab_entry(e);
+    f.add(vperm(t.t0hi, t.t0lo, s.s0));
+    f.add(vperm(t.t1hi, t.t1lo, s.s1));
+    f.add(vperm(t.t2, t.t2, s.s2));
 }
 
 // l * v for the byte constant whose tables start at e.
 __device__ __forceinline__ uint32_t cmul(const uint32_t *__restrict__ e, const Sel &s) {
-    return xor3(vperm(e[1], e[0], s.s0), vperm(e[3], e[2], s.s1), vperm(e[4], e[4], s.s2));
+    const TabEntry t = tab_entry(e);
+    return xor3(vperm(t.t0hi, t.t0lo, s.s0), vperm(t.t1hi, t.t1lo, s.s1), vperm(t.t2, t.t2, s.s2));
 }
 
 __device__ __forceinline__ Sel sel_xor(const Sel &a, const Sel &b) {
@@ -217,15 +256,21 @@ struct TileLanes {
 // cached accesses to the same addresses saturate near 5.4 TB/s.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
+// Every stream lives in device (global) memory; saying so lets pointers that
+// were loaded from memory use global_load/store with a scalar base instead
+// of flat accesses with per-lane 64-bit address arithmetic.
+typedef __attribute__((address_space(1))) u32x4 g_u32x4;
+
 __device__ __forceinline__ void load16(uint32_t *x, const uint8_t *p) {
-    const u32x4 v = __builtin_nontemporal_load((const u32x4 *)p);
+    const u32x4 v = __builtin_nontemporal_load((const g_u32x4 *)p);
     x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
 }
 
 __device__ __forceinline__ void store16(uint8_t *p, const uint32_t *w) {
     const u32x4 v = {w[0], w[1], w[2], w[3]};
-    __builtin_nontemporal_store(v, (u32x4 *)p);
+    __builtin_nontemporal_store(v, (g_u32x4 *)p);
 }
+
 
 template <int D, typename SrcPtrs>
 __device__ __forceinline__ void load_source(uint32_t (&x)[D], SrcPtrs src, uint32_t t, uint64_t base,
@@ -281,7 +326,7 @@ struct LdsTab {
     __device__ __forceinline__ void fold(Fold &f, int s, int j, int i, const Sel &sel) const {
         const int r = (s * m + j) * E + i;
         const uint4 e = lt[r];
-        const uint32_t t2 = T2LDS ? l2[r] : tt[s * stride_t + j * stride_j + i * TAB_DW + 4];
+        const uint32_t t2 = T2LDS ? l2[r] : ld_const(tt + s * stride_t + j * stride_j + i * TAB_DW + 4);
         f.add(vperm(e.y, e.x, sel.s0));
         f.add(vperm(e.w, e.z, sel.s1));
         f.add(vperm(t2, t2, sel.s2));
@@ -289,7 +334,7 @@ struct LdsTab {
     __device__ __forceinline__ uint32_t lookup(int s, int j, int i, const Sel &sel) const {
         const int r = (s * m + j) * E + i;
         const uint4 e = lt[r];
-        const uint32_t t2 = T2LDS ? l2[r] : tt[s * stride_t + j * stride_j + i * TAB_DW + 4];
+        const uint32_t t2 = T2LDS ? l2[r] : ld_const(tt + s * stride_t + j * stride_j + i * TAB_DW + 4);
         return xor3(vperm(e.y, e.x, sel.s0), vperm(e.w, e.z, sel.s1), vperm(t2, t2, sel.s2));
     }
 };
@@ -485,6 +530,102 @@ __device__ __forceinline__ void matrix_tile(SrcPtrs src, DstPtrs dst, const Tab 
 #pragma unroll
         for (int i = 0; i < P; ++i)
             if (tl.ok[i]) store16(dst[j] + base + tl.lo[i], &w[4 * i]);
+    }
+}
+
+// A run of nsub consecutive wave tiles of one operation (tile0, tile0 +
+// 256*D, ...), rows[j] (=|^=) sum_t c(t, j) * src[t], as ONE software
+// pipeline. Each tile is a sequence of steps: the k sources, then (when
+// accumulating) the M destination rows, each step one D-dword stream load
+// that lands in a ring of R register slots. R - 1 loads are in flight while
+// one slot is folded in, and the ring runs straight across tile boundaries,
+// so neither a new tile nor the destination seed of an accumulate waits
+// for a fresh load. A source step folds one source into every row through
+// the tables; a destination step XORs the loaded row into its accumulator;
+// after a tile's last step its M rows are stored and the accumulators
+// cleared. Columns at or past ncols are neither loaded nor stored. All
+// pointers must be 16-byte aligned and ncols a multiple of 16.
+template <int B, int M, int D, bool FP, int R, typename SrcPtrs, typename DstPtrs, typename Tab>
+__device__ __forceinline__ void matrix_run(SrcPtrs src, DstPtrs dst, const Tab &tab, uint32_t k,
+                                           bool ow, uint64_t ncols, uint64_t tile0, uint32_t nsub,
+                                           uint32_t lane) {
+    constexpr int P = D / 4;
+    constexpr int G = D / B;
+    constexpr uint64_t CHUNK = 256u * D;
+    static_assert(D % 4 == 0 && D % B == 0 && R >= 2, "run shape");
+    const uint32_t per = k + (ow ? 0u : (uint32_t)M);
+    const uint32_t nsteps = nsub * per;
+
+    uint32_t lo[P];
+#pragma unroll
+    for (int i = 0; i < P; ++i) lo[i] = (uint32_t)i * 1024u + lane * 16u;
+
+    // Issue cursor: step, tile, step within the tile.
+    uint32_t is = 0, ic = 0, it = 0;
+    auto issue = [&](uint32_t (&x)[D]) {
+        if (is < nsteps) {
+            const uint64_t base = tile0 + (uint64_t)ic * CHUNK;
+            const uint8_t *p = it < k ? (const uint8_t *)src[it] : (const uint8_t *)dst[it - k];
+#pragma unroll
+            for (int i = 0; i < P; ++i) {
+                x[4 * i + 0] = x[4 * i + 1] = x[4 * i + 2] = x[4 * i + 3] = 0;
+                if (base + lo[i] < ncols) load16(&x[4 * i], p + base + lo[i]);
+            }
+        }
+        ++is;
+        if (++it == per) {
+            it = 0;
+            ++ic;
+        }
+    };
+
+    uint32_t ring[R][D];
+#pragma unroll
+    for (int r = 0; r < R; ++r) issue(ring[r]);
+
+    uint32_t acc[M][D];
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+#pragma unroll
+        for (int d = 0; d < D; ++d) acc[j][d] = 0;
+
+    uint32_t cs = 0, cc = 0, ct = 0;  // consume cursor
+    while (cs < nsteps) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (cs >= nsteps) break;
+            if (ct < k) {
+                consume<B, M, D, 1, false, FP>(acc, ring[r], ring[r], tab.at(ct), M);
+            } else {
+                const uint32_t jd = ct - k;
+                uint32_t pl[D];
+#pragma unroll
+                for (int g = 0; g < G; ++g) to_planes<B>(&ring[r][g * B], &pl[g * B]);
+#pragma unroll
+                for (int j = 0; j < M; ++j)
+                    if (jd == (uint32_t)j)
+#pragma unroll
+                        for (int d = 0; d < D; ++d) acc[j][d] ^= pl[d];
+            }
+            issue(ring[r]);
+            ++cs;
+            if (++ct == per) {
+                const uint64_t base = tile0 + (uint64_t)cc * CHUNK;
+#pragma unroll
+                for (int j = 0; j < M; ++j) {
+                    uint32_t w[D];
+#pragma unroll
+                    for (int g = 0; g < G; ++g) from_planes<B>(&acc[j][g * B], &w[g * B]);
+#pragma unroll
+                    for (int i = 0; i < P; ++i)
+                        if (base + lo[i] < ncols) store16((uint8_t *)dst[j] + base + lo[i], &w[4 * i]);
+#pragma unroll
+                    for (int d = 0; d < D; ++d) acc[j][d] = 0;
+                }
+                ct = 0;
+                ++cc;
+            }
+        }
     }
 }
 
