@@ -17,7 +17,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libfgf_amd.so")
 
-SOURCES = ["kernels.hip", "api.hip", "batch.hip", "serve.hip"]
+SOURCES = ["kernels.hip", "elementwise.hip", "api.hip", "batch.hip", "serve.hip"]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",

@@ -49,7 +49,7 @@ __device__ __forceinline__ void item_tiles(const StripeDesc &s, uint32_t j0, uin
     const uint32_t nsub = (uint32_t)min<uint64_t>(subtiles, left);
     core::matrix_run<B, MC, D, FP, ring_slots(B)>(core::ConstPtrs<const uint8_t>{s.srcs},
                                                   core::ConstPtrs<uint8_t>{s.rows + j0}, tab, s.k,
-                                                  s.overwrite != 0, s.row_len, tile, nsub, lane);
+                                                  s.overwrite != 0, 0, s.row_len, tile, nsub, lane);
 }
 
 template <int B, bool FP, int D, int... Ms>

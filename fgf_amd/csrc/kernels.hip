@@ -6,8 +6,7 @@
 // pieces, plus a byte-wise generic kernel for heads, tails and unaligned
 // buffers.
 //
-// Elementwise family (mul_elementwise): products through discrete-log tables
-// of the byte field held in LDS, composed by the tower recursions.
+// The elementwise family (mul_elementwise) is in elementwise.hip.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -318,147 +317,10 @@ hipError_t launch_matrix(const MatrixJob &job) {
 }
 
 // ---------------------------------------------------------------------------
-// Elementwise products.
+// Zero fill (dot_product_matrix with no terms, mul_into(0), mul_assign(0)).
 // ---------------------------------------------------------------------------
 
 namespace {
-
-__constant__ ByteLogTables c_log_tables[3] = {make_log_tables(GF8B), make_log_tables(GF8D),
-                                              make_log_tables(FP8)};
-
-struct DevLog {
-    const uint8_t *ex;
-    const uint8_t *lg;
-    __device__ __forceinline__ uint32_t mul(uint32_t a, uint32_t b) const {
-        return (a && b) ? ex[lg[a] + lg[b]] : 0u;
-    }
-};
-
-// Karatsuba towers over Gf8B (gf16.rs:143-153, gf32.rs:121-129, gf64.rs:120-128).
-template <unsigned BITS>
-__device__ __forceinline__ uint64_t tower_mul(const DevLog &t, uint64_t x, uint64_t y) {
-    if constexpr (BITS == 8) {
-        return t.mul((uint32_t)x, (uint32_t)y);
-    } else {
-        constexpr unsigned H = BITS / 2;
-        constexpr uint64_t MASK = (1ull << H) - 1;
-        constexpr uint64_t DELTA = BITS == 16 ? 0x20u : BITS == 32 ? 0x2000u : 0x20000000ull;
-        const uint64_t x0 = x & MASK, x1 = x >> H, y0 = y & MASK, y1 = y >> H;
-        const uint64_t p00 = tower_mul<H>(t, x0, y0), p11 = tower_mul<H>(t, x1, y1);
-        const uint64_t lo = p00 ^ tower_mul<H>(t, DELTA, p11);
-        const uint64_t hi = tower_mul<H>(t, x0 ^ x1, y0 ^ y1) ^ p00;
-        return lo | (hi << H);
-    }
-}
-
-// Canonical Fan-Paar towers over fp8 (fan_paar.rs:124-145).
-template <unsigned BITS>
-__device__ __forceinline__ uint64_t fp_mul(const DevLog &t, uint64_t x, uint64_t y) {
-    if constexpr (BITS == 8) {
-        return t.mul((uint32_t)x, (uint32_t)y);
-    } else {
-        constexpr unsigned H = BITS / 2;
-        constexpr uint64_t MASK = (1ull << H) - 1;
-        const uint64_t x0 = x & MASK, x1 = x >> H, y0 = y & MASK, y1 = y >> H;
-        const uint64_t lo = fp_mul<H>(t, x0, y0), hi = fp_mul<H>(t, x1, y1);
-        const uint64_t mid = fp_mul<H>(t, x0 ^ x1, y0 ^ y1) ^ lo ^ hi;
-        return (lo ^ hi) | ((mid ^ fp_alpha(hi, H)) << H);
-    }
-}
-
-template <int F>
-__device__ __forceinline__ uint64_t elem_mul(const DevLog &t, uint64_t x, uint64_t y) {
-    constexpr unsigned BITS = 8u * field_bytes(F);
-    if constexpr (F == GF8B || F == GF8D || F == FP8) return t.mul((uint32_t)x, (uint32_t)y);
-    else if constexpr (is_fan_paar(F)) return fp_mul<BITS>(t, x, y);
-    else return tower_mul<BITS>(t, x, y);
-}
-
-template <int F>
-__device__ __forceinline__ void load_log_tables(uint8_t *s_ex, uint8_t *s_lg) {
-    constexpr int BF = byte_field(F);
-    constexpr int idx = BF == GF8B ? 0 : BF == GF8D ? 1 : 2;
-    const ByteLogTables &src = c_log_tables[idx];
-    for (int i = threadIdx.x; i < 512; i += BLOCK) s_ex[i] = src.exp[i];
-    for (int i = threadIdx.x; i < 256; i += BLOCK) s_lg[i] = src.log[i];
-    __syncthreads();
-}
-
-// 16 aligned bytes per thread per step; grid-stride over [0, n16) pieces.
-template <int F>
-__global__ __launch_bounds__(BLOCK) void k_elementwise(uint8_t *dst, const uint8_t *a,
-                                                        const uint8_t *b, uint64_t n16) {
-    __shared__ uint8_t s_ex[512], s_lg[256];
-    load_log_tables<F>(s_ex, s_lg);
-    const DevLog t{s_ex, s_lg};
-    constexpr int NB = field_bytes(F);
-    for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n16;
-         i += (uint64_t)gridDim.x * BLOCK) {
-        const uint4 va = ((const uint4 *)a)[i], vb = ((const uint4 *)b)[i];
-        uint32_t xa[4] = {va.x, va.y, va.z, va.w}, xb[4] = {vb.x, vb.y, vb.z, vb.w}, r[4];
-        if constexpr (NB == 1) {
-#pragma unroll
-            for (int d = 0; d < 4; ++d) {
-                uint32_t o = 0;
-#pragma unroll
-                for (int k = 0; k < 4; ++k)
-                    o |= t.mul((xa[d] >> (8 * k)) & 0xFF, (xb[d] >> (8 * k)) & 0xFF) << (8 * k);
-                r[d] = o;
-            }
-        } else if constexpr (NB == 2) {
-#pragma unroll
-            for (int d = 0; d < 4; ++d) {
-                const uint32_t lo = (uint32_t)elem_mul<F>(t, xa[d] & 0xFFFF, xb[d] & 0xFFFF);
-                const uint32_t hi = (uint32_t)elem_mul<F>(t, xa[d] >> 16, xb[d] >> 16);
-                r[d] = lo | (hi << 16);
-            }
-        } else if constexpr (NB == 4) {
-#pragma unroll
-            for (int d = 0; d < 4; ++d) r[d] = (uint32_t)elem_mul<F>(t, xa[d], xb[d]);
-        } else {
-#pragma unroll
-            for (int d = 0; d < 4; d += 2) {
-                const uint64_t p = elem_mul<F>(t, xa[d] | ((uint64_t)xa[d + 1] << 32),
-                                               xb[d] | ((uint64_t)xb[d + 1] << 32));
-                r[d] = (uint32_t)p;
-                r[d + 1] = (uint32_t)(p >> 32);
-            }
-        }
-        ((uint4 *)dst)[i] = make_uint4(r[0], r[1], r[2], r[3]);
-    }
-}
-
-// One element per thread with byte loads: heads, tails, unaligned buffers.
-template <int F>
-__global__ __launch_bounds__(BLOCK) void k_elementwise_generic(uint8_t *dst, const uint8_t *a,
-                                                                const uint8_t *b, uint64_t nelems) {
-    __shared__ uint8_t s_ex[512], s_lg[256];
-    load_log_tables<F>(s_ex, s_lg);
-    const DevLog t{s_ex, s_lg};
-    constexpr int NB = field_bytes(F);
-    for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < nelems;
-         i += (uint64_t)gridDim.x * BLOCK) {
-        uint64_t x = 0, y = 0;
-        for (int k = 0; k < NB; ++k) {
-            x |= (uint64_t)a[i * NB + k] << (8 * k);
-            y |= (uint64_t)b[i * NB + k] << (8 * k);
-        }
-        const uint64_t p = elem_mul<F>(t, x, y);
-        for (int k = 0; k < NB; ++k) dst[i * NB + k] = (uint8_t)(p >> (8 * k));
-    }
-}
-
-using EwFn = void (*)(uint8_t *, const uint8_t *, const uint8_t *, uint64_t);
-
-template <int... Fs>
-void pick_ew(int field, bool vector, EwFn &fn, std::integer_sequence<int, Fs...>) {
-    ((field == Fs ? (fn = vector ? (EwFn)k_elementwise<Fs> : (EwFn)k_elementwise_generic<Fs>, 0) : 0), ...);
-}
-
-unsigned grid_for(uint64_t items) {
-    const uint64_t blocks = (items + BLOCK - 1) / BLOCK;
-    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(blocks, 256u * 16u));
-}
 
 __global__ __launch_bounds__(BLOCK) void k_fill_zero(uint8_t *dst, uint64_t len) {
     // Pieces of 16 bytes aligned to the buffer's 16-byte grid.
@@ -476,38 +338,12 @@ __global__ __launch_bounds__(BLOCK) void k_fill_zero(uint8_t *dst, uint64_t len)
     }
 }
 
-}  // namespace
-
-hipError_t launch_elementwise(int field, uint8_t *dst, const uint8_t *a, const uint8_t *b,
-                              size_t len, hipStream_t stream) {
-    if (len == 0) return hipSuccess;
-    const int nb = field_bytes(field);
-    const uintptr_t mis = (uintptr_t)dst & 15u;
-    uint64_t head = len, body = 0;
-    if (((uintptr_t)a & 15u) == mis && ((uintptr_t)b & 15u) == mis) {
-        const uint64_t h = (16u - mis) & 15u;
-        if (h % nb == 0 && h <= len) {
-            head = h;
-            body = ((len - h) / 16u) * 16u;
-        }
-    }
-    const uint64_t tail0 = head + body, tail = len - tail0;
-    EwFn vec = nullptr, gen = nullptr;
-    pick_ew(field, true, vec, std::make_integer_sequence<int, FIELD_COUNT>{});
-    pick_ew(field, false, gen, std::make_integer_sequence<int, FIELD_COUNT>{});
-    if (head) {
-        hipLaunchKernelGGL(gen, dim3(grid_for(head / nb)), dim3(BLOCK), 0, stream, dst, a, b, head / nb);
-    }
-    if (body) {
-        hipLaunchKernelGGL(vec, dim3(grid_for(body / 16)), dim3(BLOCK), 0, stream, dst + head,
-                           a + head, b + head, body / 16);
-    }
-    if (tail) {
-        hipLaunchKernelGGL(gen, dim3(grid_for(tail / nb)), dim3(BLOCK), 0, stream, dst + tail0,
-                           a + tail0, b + tail0, tail / nb);
-    }
-    return hipGetLastError();
+unsigned grid_for(uint64_t items) {
+    const uint64_t blocks = (items + BLOCK - 1) / BLOCK;
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(blocks, 256u * 16u));
 }
+
+}  // namespace
 
 hipError_t launch_fill_zero(uint8_t *dst, size_t len, hipStream_t stream) {
     if (len == 0) return hipSuccess;

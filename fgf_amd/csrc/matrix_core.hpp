@@ -543,12 +543,13 @@ __device__ __forceinline__ void matrix_tile(SrcPtrs src, DstPtrs dst, const Tab 
 // for a fresh load. A source step folds one source into every row through
 // the tables; a destination step XORs the loaded row into its accumulator;
 // after a tile's last step its M rows are stored and the accumulators
-// cleared. Columns at or past ncols are neither loaded nor stored. All
-// pointers must be 16-byte aligned and ncols a multiple of 16.
+// cleared. Columns are relative to col0; those at or past ncols are neither
+// loaded nor stored. Every pointer + col0 must be 16-byte aligned and ncols
+// a multiple of 16.
 template <int B, int M, int D, bool FP, int R, typename SrcPtrs, typename DstPtrs, typename Tab>
 __device__ __forceinline__ void matrix_run(SrcPtrs src, DstPtrs dst, const Tab &tab, uint32_t k,
-                                           bool ow, uint64_t ncols, uint64_t tile0, uint32_t nsub,
-                                           uint32_t lane) {
+                                           bool ow, uint64_t col0, uint64_t ncols, uint64_t tile0,
+                                           uint32_t nsub, uint32_t lane) {
     constexpr int P = D / 4;
     constexpr int G = D / B;
     constexpr uint64_t CHUNK = 256u * D;
@@ -565,7 +566,7 @@ __device__ __forceinline__ void matrix_run(SrcPtrs src, DstPtrs dst, const Tab &
     auto issue = [&](uint32_t (&x)[D]) {
         if (is < nsteps) {
             const uint64_t base = tile0 + (uint64_t)ic * CHUNK;
-            const uint8_t *p = it < k ? (const uint8_t *)src[it] : (const uint8_t *)dst[it - k];
+            const uint8_t *p = (it < k ? (const uint8_t *)src[it] : (const uint8_t *)dst[it - k]) + col0;
 #pragma unroll
             for (int i = 0; i < P; ++i) {
                 x[4 * i + 0] = x[4 * i + 1] = x[4 * i + 2] = x[4 * i + 3] = 0;
@@ -618,7 +619,7 @@ __device__ __forceinline__ void matrix_run(SrcPtrs src, DstPtrs dst, const Tab &
                     for (int g = 0; g < G; ++g) from_planes<B>(&acc[j][g * B], &w[g * B]);
 #pragma unroll
                     for (int i = 0; i < P; ++i)
-                        if (base + lo[i] < ncols) store16((uint8_t *)dst[j] + base + lo[i], &w[4 * i]);
+                        if (base + lo[i] < ncols) store16((uint8_t *)dst[j] + col0 + base + lo[i], &w[4 * i]);
 #pragma unroll
                     for (int d = 0; d < D; ++d) acc[j][d] = 0;
                 }

@@ -393,6 +393,18 @@ def test_elementwise_matches_field_arithmetic(f):
         assert np.array_equal(host(got), ol.mul_elementwise(name, a, b)), length
 
 
+@pytest.mark.parametrize("f", [Gf8D, Gf64, FanPaar32], ids=lambda f: NAMES[f.id])
+def test_elementwise_large_buffer_wraps_the_grid(f):
+    # More than one pass of the persistent grid (512 blocks x 8 waves x
+    # 2 KiB), plus a tail that is not a whole wave tile.
+    name = NAMES[f.id]
+    length = (17 << 20) + 48
+    a, b = noise(length, 0x356), noise(length, 0x357)
+    got = dev(np.zeros(length, np.uint8))
+    ops.mul_elementwise(f, got, dev(a), dev(b))
+    assert np.array_equal(host(got), ol.mul_elementwise(name, a, b))
+
+
 # ---------------------------------------------------------------------------
 # Tower and Fan-Paar public ops (tests/ops.rs:1027-1113)
 # ---------------------------------------------------------------------------
