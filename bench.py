@@ -181,6 +181,48 @@ def bench_batched(n_stripes: int = 4096, seed: int = 4096, reps: int = 5) -> dic
     }
 
 
+def profile_batched(launches: int, n_stripes: int = 4096, seed: int = 4096,
+                    config=(4, 8, 4, 32, 1)) -> dict:
+    """Only the batched launch of the uneven-stripe mix, `launches` times, at
+    a fixed tile config (dwords, rows per item, subtiles, waves per CU,
+    order): the run that the batch kernel's rocprofv3 counter and trace
+    profiles are taken from. The first launch's output is checked against
+    the reconstructed shards."""
+    import torch
+
+    from fgf_amd import Gf8D, ops, workloads
+    from fgf_amd.batch import Batch, Stripe
+
+    specs = workloads.draw_uneven_mix(Gf8D, n_stripes, seed)
+    pool, saved = workloads.materialize(Gf8D, specs, seed + 1)
+    plans, stripes = {}, []
+    for sp in specs:
+        key = (sp.k, len(sp.out_rows), tuple(sp.plan_values))
+        if key not in plans:
+            plans[key] = ops.Plan.matrix(Gf8D, sp.k, len(sp.out_rows), sp.plan_values)
+        srcs, rows = workloads.stripe_buffers(pool, sp)
+        stripes.append(Stripe(plans[key], srcs, rows, sp.row_len, True))
+    batch = Batch(Gf8D, stripes)
+    batch.configure(*config)
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(launches + 1)]
+    ev[0].record(stream)
+    for i in range(launches):
+        batch.run()
+        ev[i + 1].record(stream)
+    torch.cuda.synchronize()
+    recovered = all(torch.equal(g, s) for g, s, sp in
+                    zip([torch.cat(workloads.stripe_buffers(pool, sp)[1]) for sp in specs], saved, specs)
+                    if sp.kind == "reconstruct")
+    assert recovered, "batched reconstruction differs from the erased shards"
+    per = sorted(ev[i].elapsed_time(ev[i + 1]) for i in range(launches))
+    total = sum(sp.algorithmic_bytes for sp in specs)
+    return {"workload": f"{n_stripes} Gf8D stripes (uneven mix), batched launch only",
+            "config": list(config), "launches": launches, "items": batch.items,
+            "median_ms": round(per[len(per) // 2], 4), "algorithmic_bytes": total,
+            "GBps": round(total / (per[len(per) // 2] * 1e-3) / 1e9, 1)}
+
+
 def _median_launch_ms(fn, reps: int = 20, warm: int = 3) -> float:
     """Median per-launch time from HIP events recorded between back-to-back
     launches on the current stream."""
@@ -411,12 +453,18 @@ def main() -> int:
     ap.add_argument("--calib-copy", action="store_true",
                     help="after timing, run one 1 GiB device copy (a plain copy kernel of known "
                          "size, for scaling HBM traffic counters in a --pmc profile)")
+    ap.add_argument("--profile-batched", type=int, default=0, metavar="N",
+                    help="run only N batched launches of the uneven mix (for rocprofv3) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               f"(the sharding design is documented, multi-GPU runs are not part of it)",
               file=sys.stderr)
         return 2
+
+    if args.profile_batched > 0:
+        print(json.dumps(profile_batched(args.profile_batched)), flush=True)
+        return 0
 
     import numpy as np
     import torch
