@@ -67,7 +67,7 @@ constexpr int BLOCK = 256;
 
 // Matrix kernels read their tables from LDS (see k_matrix): k*M*kterms(B)
 // records per block, kept within LDS_TABLE_BYTES by the source chunk (76 KiB:
-// two blocks per CU, and a k = 32 chunk of GF(2^64) at four rows).
+// two blocks per CU, and a 64-source chunk at every width's row count).
 constexpr size_t LDS_TABLE_BYTES = 76 * 1024;
 // T2 dwords go to LDS too when a block's kterms(B)*M scalars per source would
 // crowd the SGPRs (towers, and byte fields with more than 8 rows).
@@ -100,8 +100,12 @@ __device__ __forceinline__ bool map_block(const MatrixArgs &a, uint32_t &ct, uin
 //
 // The block first stages its rows' tables in LDS (dynamic shared memory, at
 // most LDS_TABLE_BYTES), then every wave runs its tile.
+// GF(2^64) asks for three waves per SIMD: its 81 Karatsuba selector
+// registers would otherwise leave two, too few to cover its 27 table reads
+// per source and row (measured 11% faster at two rows per block).
 template <int B, bool FP, int M, int D, bool OW>
-__global__ __launch_bounds__(BLOCK) void k_matrix(const MatrixArgs a) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(B == 8 ? 3 : 1)))
+void k_matrix(const MatrixArgs a) {
     uint32_t ct, rg;
     if (!map_block(a, ct, rg)) return;
     const uint32_t lane = threadIdx.x & 63u;
@@ -168,7 +172,7 @@ __global__ __launch_bounds__(BLOCK) void k_matrix_generic(const MatrixArgs a) {
 using MatrixFn = void (*)(MatrixArgs);
 
 // Rows per block for each element width, bounded by the accumulator budget.
-constexpr int max_rows(int nb) { return nb == 1 ? 16 : nb == 2 ? 8 : 4; }
+constexpr int max_rows(int nb) { return nb == 1 ? 16 : nb == 2 ? 8 : nb == 4 ? 4 : 2; }
 // Dynamic LDS one block may use for tables on the current device: the
 // smaller of LDS_TABLE_BYTES and the device's per-block limit.
 size_t lds_budget() {
