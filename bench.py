@@ -425,14 +425,58 @@ def bench_serving(n_requests: int = 4096, seed: int = 77,
         })
     st = srv.stats()
     srv.close()
-    del pool
+
+    # The same requests as plain per-call launches (one dot_product_matrix per
+    # request: the parity rows of an encode, the erased shards of a rebuild
+    # through its decode matrix), back to back on one stream.
+    from fgf_amd import codec, ops
+    from fgf_amd.batch import Batch, Stripe
+    plans, groups = {}, {}
+    for sp in specs:
+        key = (sp.field.id, sp.k, sp.m, tuple(sp.erased))
+        shard = lambda i, sp=sp: workloads.serve_shard(pool, sp, i)
+        if key not in plans:
+            if sp.erased:
+                dmat, surv = codec.decode_matrix(sp.field, sp.k, sp.m, sp.erased)
+                plans[key] = (ops.Plan.matrix(sp.field, sp.k, len(sp.erased), codec.plan_values(dmat)), surv)
+            else:
+                plans[key] = (ops.Plan.matrix(sp.field, sp.k, sp.m,
+                                              codec.encode_plan_values(sp.field, sp.k, sp.m)),
+                              list(range(sp.k)))
+        plan, srcs = plans[key]
+        groups.setdefault(sp.field.id, (sp.field, []))[1].append(
+            Stripe(plan, [shard(i) for i in srcs], [shard(i) for i in sp.out_shards], sp.row_len, True))
+    batches = [Batch(f, st_) for f, st_ in groups.values()]
+    for sp in specs:
+        for i in sp.out_shards:
+            workloads.serve_shard(pool, sp, i).zero_()
+    stream = torch.cuda.current_stream()
+    times = []
+    for _ in range(3):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for bt in batches:
+            bt.run_per_call()
+        b.record(stream)
+        torch.cuda.synchronize()
+        times.append(a.elapsed_time(b))
+    t_call = sorted(times)[1]
+    ok = all(torch.equal(torch.cat([workloads.serve_shard(pool, sp, i) for i in sp.out_shards]), w)
+             for sp, w in zip(specs, expected))
+    assert ok, "per-call output differs from the expected shards"
+    per_call_gbps = total / (t_call * 1e-3) / 1e9
+    burst = [l for l in legs if l["offered_req_per_s"] == "burst"]
+    del batches, pool
     torch.cuda.empty_cache()
     return {
         "workload": f"{n_requests} requests, 80% Gf8D / 20% Gf16, 256 coefficient matrices, "
                     f"k~U[4,32], m~U[2,8], L log-uniform 4KiB-1MiB, 50% reconstruct matrices",
         "algorithmic_bytes": total,
-        "server": {"slots": 4, "hbm_budget_GiB": 64, "max_batch_stripes": 256},
+        "server": {"slots": 4, "hbm_budget_GiB": 64, "max_batch_stripes": 512,
+                   "tile": {"dwords": 4, "rows_per_item": 8, "subtiles": 4, "waves_per_cu": 8}},
         "legs": legs,
+        "per_call": {"ms": round(t_call, 3), "GBps": round(per_call_gbps, 1)},
+        "burst_speedup_vs_per_call": round(burst[0]["GBps"] / per_call_gbps, 2) if burst else None,
         "plan_cache": {"hits": st["plan_hits"], "misses": st["plan_misses"],
                        "evictions": st["plan_evictions"]},
         "bit_exact_vs_per_call": True,

@@ -140,22 +140,40 @@ std::vector<WorkItem> build_work_items(const std::vector<StripeDesc> &stripes, i
                                        int rows_per_item, int subtiles, int order) {
     const uint64_t chunk = 256u * (uint64_t)dwords * (uint64_t)subtiles;
     const uint32_t rows = (uint32_t)rows_per_item;
+    size_t n = 0;
+    for (const StripeDesc &d : stripes)
+        n += (size_t)((d.row_len + chunk - 1) / chunk) * ((d.m + rows - 1) / rows);
     std::vector<WorkItem> items;
+    items.reserve(n);
+    // With order 1 each item also gets a sort key: its cost (columns x
+    // sources x rows; below 2^26 up to 64 sources, larger costs saturate)
+    // in the high 28 bits, inverted so that larger costs sort first, and its
+    // position in stripe order in the low 36 bits, so that equal costs keep
+    // that order.
+    std::vector<uint64_t> keys;
+    if (order == 1) keys.reserve(n);
     for (uint32_t s = 0; s < stripes.size(); ++s) {
         const StripeDesc &d = stripes[s];
         const uint32_t nchunks = (uint32_t)((d.row_len + chunk - 1) / chunk);
         const uint32_t ngroups = (d.m + rows - 1) / rows;
-        for (uint32_t c = 0; c < nchunks; ++c)
-            for (uint32_t g = 0; g < ngroups; ++g) items.push_back({s, c, g, 0});
+        for (uint32_t c = 0; c < nchunks; ++c) {
+            const uint64_t cols = std::min<uint64_t>(chunk, d.row_len - (uint64_t)c * chunk);
+            for (uint32_t g = 0; g < ngroups; ++g) {
+                if (order == 1) {
+                    const uint64_t cost = std::min<uint64_t>(
+                        cols * d.k * std::min<uint32_t>(rows, d.m - g * rows), (1ull << 28) - 1);
+                    keys.push_back((((1ull << 28) - 1 - cost) << 36) | items.size());
+                }
+                items.push_back({s, c, g, 0});
+            }
+        }
     }
     if (order == 1) {
-        auto cost = [&](const WorkItem &w) {
-            const StripeDesc &d = stripes[w.stripe];
-            const uint64_t cols = std::min<uint64_t>(chunk, d.row_len - (uint64_t)w.chunk * chunk);
-            return cols * d.k * std::min<uint32_t>(rows, d.m - w.rowgroup * rows);
-        };
-        std::stable_sort(items.begin(), items.end(),
-                         [&](const WorkItem &x, const WorkItem &y) { return cost(x) > cost(y); });
+        std::sort(keys.begin(), keys.end());
+        std::vector<WorkItem> sorted;
+        sorted.reserve(items.size());
+        for (uint64_t k : keys) sorted.push_back(items[k & ((1ull << 36) - 1)]);
+        items.swap(sorted);
     }
     return items;
 }

@@ -65,10 +65,30 @@ struct Req {
     Clock::time_point submitted;
 };
 
+// Plan cache key: field, encode/reconstruct, k, m and the erased set as a
+// bitmask over the k + m <= 256 shards.
+struct PlanKey {
+    uint32_t head;        // field | reconstruct << 8 | k << 9 (k <= 64)
+    uint32_t m;
+    uint64_t erased[4];
+    bool operator==(const PlanKey &o) const {
+        return head == o.head && m == o.m && erased[0] == o.erased[0] && erased[1] == o.erased[1] &&
+               erased[2] == o.erased[2] && erased[3] == o.erased[3];
+    }
+};
+
+struct PlanKeyHash {
+    size_t operator()(const PlanKey &k) const {
+        uint64_t h = ((uint64_t)k.head << 32 | k.m) * 0x9E3779B97F4A7C15ull;
+        for (uint64_t e : k.erased) h = (h ^ e) * 0x9E3779B97F4A7C15ull;
+        return (size_t)(h ^ (h >> 29));
+    }
+};
+
 struct PlanEntry {
     fgf_plan *plan = nullptr;
     uint32_t inflight = 0;
-    std::list<std::string>::iterator lru;
+    std::list<PlanKey>::iterator lru;
 };
 
 struct Slot {
@@ -94,15 +114,18 @@ struct fgf_server {
     std::vector<Slot> slots;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
-    std::deque<Req> queue;
+    // Queued requests per field, each in arrival (ticket) order; the head
+    // request overall is the front with the smallest ticket.
+    std::deque<Req> queues[fgf::FIELD_COUNT];
+    size_t queued = 0;
     bool stop = false;
     uint64_t next_ticket = 0;
     uint64_t inflight_bytes = 0;
     std::vector<double> latency;   // per ticket, -1 while pending
     std::vector<int> status;
     fgf_server_stats stats{};
-    std::unordered_map<std::string, PlanEntry> cache;
-    std::list<std::string> lru;
+    std::unordered_map<PlanKey, PlanEntry, PlanKeyHash> cache;
+    std::list<PlanKey> lru;
     std::thread worker;
 };
 
@@ -162,10 +185,9 @@ std::vector<uint32_t> survivors_of(const Req &q) {
     return s;
 }
 
-std::string plan_key(const Req &q) {
-    std::string k = std::to_string(q.r.field) + (q.r.reconstruct ? "R" : "E") + std::to_string(q.r.k) +
-                    "," + std::to_string(q.r.m);
-    for (uint32_t e : q.erased) k += ":" + std::to_string(e);
+PlanKey plan_key(const Req &q) {
+    PlanKey k{(uint32_t)q.r.field | (q.r.reconstruct ? 1u : 0u) << 8 | q.r.k << 9, q.r.m, {0, 0, 0, 0}};
+    for (uint32_t e : q.erased) k.erased[e >> 6] |= 1ull << (e & 63);
     return k;
 }
 
@@ -202,7 +224,7 @@ bool plan_values(const Req &q, std::vector<uint64_t> &vals, uint32_t &rows) {
 }
 
 PlanEntry *acquire_plan(fgf_server *s, const Req &q) {
-    const std::string key = plan_key(q);
+    const PlanKey key = plan_key(q);
     auto it = s->cache.find(key);
     if (it != s->cache.end()) {
         s->lru.splice(s->lru.begin(), s->lru, it->second.lru);
@@ -224,7 +246,7 @@ PlanEntry *acquire_plan(fgf_server *s, const Req &q) {
     // Evict least-recently-used plans that no launch is reading.
     while (s->cache.size() >= s->cfg.plan_cache_entries) {
         auto victim = std::find_if(s->lru.rbegin(), s->lru.rend(),
-                                   [&](const std::string &k2) { return s->cache[k2].inflight == 0; });
+                                   [&](const PlanKey &k2) { return s->cache[k2].inflight == 0; });
         if (victim == s->lru.rend()) break;
         auto ce = s->cache.find(*victim);
         fgf_plan_free(ce->second.plan);
@@ -357,39 +379,44 @@ void scheduler(fgf_server *s) {
                 break;
             }
         bool launched = false;
-        if (free_slot && !s->queue.empty()) {
-            const int field = s->queue.front().r.field;
-            for (auto it = s->queue.begin();
-                 it != s->queue.end() && free_slot->reqs.size() < s->cfg.max_batch_stripes;) {
-                if (it->r.field != field) {
-                    ++it;
-                    continue;
-                }
-                if (s->inflight_bytes + it->bytes > s->cfg.hbm_budget_bytes) {
+        if (free_slot && s->queued) {
+            // The head request's field: the queue front with the oldest ticket.
+            int field = -1;
+            for (int f = 0; f < FIELD_COUNT; ++f)
+                if (!s->queues[f].empty() &&
+                    (field < 0 || s->queues[f].front().ticket < s->queues[field].front().ticket))
+                    field = f;
+            std::deque<Req> &fq = s->queues[field];
+            while (!fq.empty() && free_slot->reqs.size() < s->cfg.max_batch_stripes) {
+                Req &q = fq.front();
+                if (s->inflight_bytes + q.bytes > s->cfg.hbm_budget_bytes) {
                     if (free_slot->reqs.empty() && !any_busy) {
                         // Larger than the whole budget: can never be admitted.
-                        s->latency[it->ticket] = 0;
-                        s->status[it->ticket] = FGF_ENOMEM;
+                        s->latency[q.ticket] = 0;
+                        s->status[q.ticket] = FGF_ENOMEM;
                         ++s->stats.failed;
-                        it = s->queue.erase(it);
+                        fq.pop_front();
+                        --s->queued;
                         continue;
                     }
                     ++s->stats.admission_waits;
                     break;  // in-order admission: wait for budget
                 }
-                PlanEntry *pe = acquire_plan(s, *it);
+                PlanEntry *pe = acquire_plan(s, q);
                 if (!pe) {
-                    s->latency[it->ticket] = 0;
-                    s->status[it->ticket] = FGF_EINVAL;
+                    s->latency[q.ticket] = 0;
+                    s->status[q.ticket] = FGF_EINVAL;
                     ++s->stats.failed;
-                    it = s->queue.erase(it);
+                    fq.pop_front();
+                    --s->queued;
                     continue;
                 }
-                s->inflight_bytes += it->bytes;
-                free_slot->bytes += it->bytes;
+                s->inflight_bytes += q.bytes;
+                free_slot->bytes += q.bytes;
                 free_slot->plans.push_back(pe);
-                free_slot->reqs.push_back(std::move(*it));
-                it = s->queue.erase(it);
+                free_slot->reqs.push_back(std::move(q));
+                fq.pop_front();
+                --s->queued;
             }
             s->stats.max_inflight_bytes = std::max(s->stats.max_inflight_bytes, s->inflight_bytes);
             if (!free_slot->reqs.empty()) {
@@ -406,7 +433,7 @@ void scheduler(fgf_server *s) {
                 launched = true;
             }
         }
-        if (s->stop && s->queue.empty() && !any_busy && !launched) break;
+        if (s->stop && !s->queued && !any_busy && !launched) break;
         if (!launched) {
             // Idle or waiting on the device: poll completions at a fine grain.
             if (any_busy)
@@ -439,12 +466,20 @@ int fgf_server_create(const fgf_server_config *cfg, fgf_server **out) {
     (void)hipGetDevice(&s->device);
     (void)hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, s->device);
     s->slots.resize(cfg->slots);
+    // Descriptor buffers sized for a full batch up front: growing them later
+    // frees memory, and hipFree waits for every stream of the device.
+    const size_t want_stripes = cfg->max_batch_stripes;
+    const size_t want_ptrs = (size_t)cfg->max_batch_stripes * 32;
+    const size_t want_items = (size_t)cfg->max_batch_stripes * 128;
     for (Slot &sl : s->slots) {
         hipError_t e;
         if ((e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking)) ||
             (e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming)) ||
             (e = hipMalloc((void **)&sl.d_counter, 2 * sizeof(uint32_t))) ||
-            (e = hipMemset(sl.d_counter, 0, 2 * sizeof(uint32_t)))) {
+            (e = hipMemset(sl.d_counter, 0, 2 * sizeof(uint32_t))) ||
+            (e = grow(sl.d_stripes, sl.h_stripes, sl.cap_stripes, want_stripes)) ||
+            (e = grow(sl.d_ptrs, sl.h_ptrs, sl.cap_ptrs, want_ptrs)) ||
+            (e = grow(sl.d_items, sl.h_items, sl.cap_items, want_items))) {
             fgf_server_destroy(s);
             return sdev("fgf_server_create", e);
         }
@@ -454,7 +489,12 @@ int fgf_server_create(const fgf_server_config *cfg, fgf_server **out) {
     return FGF_OK;
 }
 
-int fgf_server_submit(fgf_server *s, const fgf_request *r, uint64_t *ticket) {
+}  // extern "C"
+
+namespace {
+
+// Validates a request and builds its queue entry (no lock held).
+int make_req(fgf_server *s, const fgf_request *r, Req &q) {
     if (!s || !r) return sfail("fgf_server_submit: null argument");
     const int f = r->field;
     if (f != FGF_GF8B && f != FGF_GF8D && f != FGF_GF16 && f != FGF_FP8 && f != FGF_FP16)
@@ -467,7 +507,6 @@ int fgf_server_submit(fgf_server *s, const fgf_request *r, uint64_t *ticket) {
         return sfail("fgf_server_submit: row_len %zu is not a nonzero multiple of 16", r->row_len);
     for (uint32_t i = 0; i < r->k + r->m; ++i)
         if ((uintptr_t)r->shards[i] & 15u) return sfail("fgf_server_submit: shard %u is not 16-byte aligned", i);
-    Req q;
     q.r = *r;
     q.shards.assign(r->shards, r->shards + r->k + r->m);
     if (r->reconstruct) {
@@ -482,17 +521,39 @@ int fgf_server_submit(fgf_server *s, const fgf_request *r, uint64_t *ticket) {
     q.r.shards = nullptr;
     q.r.erased = nullptr;
     q.bytes = (uint64_t)(r->k + r->m) * r->row_len;
-    q.submitted = Clock::now();
+    return FGF_OK;
+}
+
+// Enqueues built requests in order under one lock; each gets its ticket
+// and its submission time.
+void enqueue(fgf_server *s, Req *qs, size_t n, uint64_t *tickets) {
+    const Clock::time_point now = Clock::now();
     {
         std::lock_guard<std::mutex> lk(s->mu);
-        q.ticket = s->next_ticket++;
-        s->latency.push_back(-1.0);
-        s->status.push_back(0);
-        ++s->stats.submitted;
-        if (ticket) *ticket = q.ticket;
-        s->queue.push_back(std::move(q));
+        for (size_t i = 0; i < n; ++i) {
+            Req &q = qs[i];
+            q.submitted = now;
+            q.ticket = s->next_ticket++;
+            s->latency.push_back(-1.0);
+            s->status.push_back(0);
+            ++s->stats.submitted;
+            if (tickets) tickets[i] = q.ticket;
+            s->queues[q.r.field].push_back(std::move(q));
+            ++s->queued;
+        }
     }
     s->cv_work.notify_one();
+}
+
+}  // namespace
+
+extern "C" {
+
+int fgf_server_submit(fgf_server *s, const fgf_request *r, uint64_t *ticket) {
+    Req q;
+    const int rc = make_req(s, r, q);
+    if (rc != FGF_OK) return rc;
+    enqueue(s, &q, 1, ticket);
     return FGF_OK;
 }
 
@@ -526,18 +587,35 @@ int fgf_server_replay(fgf_server *s, const fgf_request *reqs, const double *arri
     std::vector<double> lateness(n, 0.0);
     int rc = FGF_OK;
     const Clock::time_point t0 = Clock::now();
+    auto due_at = [&](size_t i) {
+        return t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(arrival[i]));
+    };
     std::thread client([&] {
-        for (size_t i = 0; i < n; ++i) {
-            const Clock::time_point due =
-                t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(arrival[i]));
-            while (Clock::now() < due) {
+        std::vector<Req> group;
+        std::vector<size_t> index;
+        std::vector<uint64_t> got;
+        for (size_t i = 0; i < n;) {
+            while (Clock::now() < due_at(i)) {
             }
-            lateness[i] = std::chrono::duration<double>(Clock::now() - due).count();
-            const int r = fgf_server_submit(s, &reqs[i], &tickets[i]);
-            if (r != FGF_OK) {
-                rc = r;
-                tickets[i] = UINT64_MAX;
+            // Every request whose arrival time has passed goes in one group.
+            const Clock::time_point now = Clock::now();
+            group.clear();
+            index.clear();
+            for (; i < n && due_at(i) <= now; ++i) {
+                lateness[i] = std::chrono::duration<double>(now - due_at(i)).count();
+                Req q;
+                const int r = make_req(s, &reqs[i], q);
+                if (r != FGF_OK) {
+                    rc = r;
+                    tickets[i] = UINT64_MAX;
+                    continue;
+                }
+                group.push_back(std::move(q));
+                index.push_back(i);
             }
+            got.resize(group.size());
+            enqueue(s, group.data(), group.size(), got.data());
+            for (size_t g = 0; g < index.size(); ++g) tickets[index[g]] = got[g];
         }
     });
     client.join();
