@@ -90,6 +90,41 @@ def test_batch_accumulate_mode_and_wide_fields():
         assert np.array_equal(rows.cpu().numpy(), exp)
 
 
+@pytest.mark.parametrize("field", [Gf8D, Gf16], ids=["gf8d", "gf16"])
+def test_batch_mixed_accumulate_and_overwrite_across_tile_shapes(field):
+    # Accumulating stripes load their destination rows as pipeline steps
+    # after the sources of every tile; with several row groups per stripe and
+    # several tiles per item those steps interleave with tile ends.
+    name = "gf8d" if field is Gf8D else "gf16"
+    g = np.random.default_rng(21)
+    host = []
+    for s in range(24):
+        k = int(g.integers(1, 20))
+        m = int(g.integers(1, 9))
+        L = int(g.integers(1, 40)) * 16 * (1 + s % 5) * 8 + 16 * (s % 3)
+        L -= L % 16
+        srcs = [g.integers(0, 256, L, dtype=np.uint8) for _ in range(k)]
+        vals = [int(v) for v in g.integers(0, 1 << field.BITS, k * m, dtype=np.uint64)]
+        init = g.integers(0, 256, m * L, dtype=np.uint8)
+        host.append((k, m, L, srcs, vals, init, s % 2 == 0))
+    for cfg in [(4, 4, 1, 8, 0), (4, 8, 4, 16, 1), (8, 4, 16, 32, 1), (8, 8, 2, 4, 0)]:
+        stripes, rows_dev = [], []
+        for k, m, L, srcs, vals, init, ow in host:
+            plan = ops.Plan.matrix(field, k, m, vals)
+            rows = torch.from_numpy(init.copy()).cuda()
+            rows_dev.append(rows)
+            stripes.append(Stripe(plan, [torch.from_numpy(x).cuda() for x in srcs],
+                                  [rows[j * L:(j + 1) * L] for j in range(m)], L, ow))
+        b = Batch(field, stripes)
+        b.configure(*cfg)
+        b.run()
+        torch.cuda.synchronize()
+        for (k, m, L, srcs, vals, init, ow), rows in zip(host, rows_dev):
+            exp = init.copy()
+            ol.mul_add_matrix(name, exp, L, m, vals, srcs, ow)
+            assert np.array_equal(rows.cpu().numpy(), exp), (cfg, k, m, L, ow)
+
+
 def test_batch_rejects_misaligned_and_overlapping_rows():
     plan = ops.Plan.matrix(Gf8D, 1, 2, [3, 5])
     buf = torch.zeros(8192, dtype=torch.uint8, device="cuda")
