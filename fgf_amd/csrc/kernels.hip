@@ -117,7 +117,7 @@ void k_matrix(const MatrixArgs a) {
     const auto tb = stage_lds_table<B, t2_lds(B, M)>(s_lt, tab, a.k, M, a.tab_stride_t,
                                                   a.tab_stride_j);
     if (tile >= a.ncols) return;
-    matrix_tile<B, M, D, false, FP>(a.src, a.dst + j0, tb, a.k, OW, a.col0, a.ncols, tile, lane);
+    matrix_tile<B, M, D, FP>(a.src, a.dst + j0, tb, a.k, OW, a.col0, a.ncols, tile, lane);
 }
 
 // Generic kernel for heads, tails and unaligned buffers: one thread per group

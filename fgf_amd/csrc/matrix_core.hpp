@@ -377,18 +377,16 @@ __device__ __forceinline__ LdsTab<B, T2LDS> stage_lds_table(void *lds, const uin
 // Towers (S = 1): the selectors of every group's Karatsuba planes live, then
 // per row the kterms(B) byte-constant products of each group go through the
 // Karatsuba network into the row's B accumulator planes.
-// With GUARD, only rows j < mrows are folded (row-outer, so each row's tables
-// are still loaded once): one body serves every row count up to M.
 template <int B, int M>
 constexpr bool dword_outer() { return B == 1 && M <= 6; }
 
-template <int B, int M, int D, int S, bool GUARD, bool FP, typename Tab>
+template <int B, int M, int D, int S, bool FP, typename Tab>
 __device__ __forceinline__ void consume(uint32_t (&acc)[M][D], const uint32_t *x0,
-                                        const uint32_t *x1, const Tab &tb, uint32_t mrows) {
+                                        const uint32_t *x1, const Tab &tb) {
     constexpr int G = D / B;
     if constexpr (B == 1) {
         const uint32_t *xs[2] = {x0, x1};
-        if constexpr (!GUARD && dword_outer<B, M>()) {
+        if constexpr (dword_outer<B, M>()) {
 #pragma unroll
             for (int g = 0; g < G; ++g) {
                 Sel sel[S];
@@ -410,7 +408,6 @@ __device__ __forceinline__ void consume(uint32_t (&acc)[M][D], const uint32_t *x
                 for (int g = 0; g < G; ++g) sel[s][g] = selectors(xs[s][g]);
 #pragma unroll
             for (int j = 0; j < M; ++j) {
-                if (GUARD && j >= (int)mrows) continue;
 #pragma unroll
                 for (int g = 0; g < G; ++g) {
                     Fold f{acc[j][g], 0u, false};
@@ -435,7 +432,6 @@ __device__ __forceinline__ void consume(uint32_t (&acc)[M][D], const uint32_t *x
         }
 #pragma unroll
         for (int j = 0; j < M; ++j) {
-            if (GUARD && j >= (int)mrows) continue;
             uint32_t lk[G][E];
 #pragma unroll
             for (int i = 0; i < E; ++i)
@@ -448,37 +444,34 @@ __device__ __forceinline__ void consume(uint32_t (&acc)[M][D], const uint32_t *x
 }
 
 // Folds the sources t (and t + 1 when present) held in x.
-template <int B, int M, int D, bool GUARD, bool FP, typename Tab>
+template <int B, int M, int D, bool FP, typename Tab>
 __device__ __forceinline__ void consume_step(uint32_t (&acc)[M][D], const uint32_t (&x)[2][D],
-                                             const Tab &tab, uint32_t t, uint32_t k,
-                                             uint32_t mrows) {
+                                             const Tab &tab, uint32_t t, uint32_t k) {
     const Tab tb = tab.at(t);
     if (t + 1 < k) {
         if constexpr (B == 1) {
-            consume<B, M, D, 2, GUARD, FP>(acc, x[0], x[1], tb, mrows);
+            consume<B, M, D, 2, FP>(acc, x[0], x[1], tb);
         } else {
-            consume<B, M, D, 1, GUARD, FP>(acc, x[0], x[0], tb, mrows);
-            consume<B, M, D, 1, GUARD, FP>(acc, x[1], x[1], tab.at(t + 1), mrows);
+            consume<B, M, D, 1, FP>(acc, x[0], x[0], tb);
+            consume<B, M, D, 1, FP>(acc, x[1], x[1], tab.at(t + 1));
         }
     } else {
-        consume<B, M, D, 1, GUARD, FP>(acc, x[0], x[0], tb, mrows);
+        consume<B, M, D, 1, FP>(acc, x[0], x[0], tb);
     }
 }
 
 // One wave's tile of rows[j] (=|^=) sum_t c(t, j) * src[t], for columns
 // [col0 + tile, col0 + tile + 256*D) clipped to col0 + ncols. All pointers
-// + col0 must be 16-byte aligned and ncols a multiple of 16. Without GUARD
-// exactly M rows are processed (a compile-time count keeps every table load
-// hoistable out of the dword loop); with GUARD rows j < mrows (<= M) are.
-// `ow` selects overwrite (zero seed) or accumulate
+// + col0 must be 16-byte aligned and ncols a multiple of 16. Exactly M rows
+// are processed (a compile-time count keeps every table load hoistable out
+// of the dword loop). `ow` selects overwrite (zero seed) or accumulate
 // (seed from the destination). Sources are consumed in pairs from two
 // ping-pong register buffers: the next pair's loads are in flight while the
 // current pair is multiplied.
-template <int B, int M, int D, bool GUARD = false, bool FP = false, typename SrcPtrs,
-          typename DstPtrs, typename Tab>
+template <int B, int M, int D, bool FP = false, typename SrcPtrs, typename DstPtrs, typename Tab>
 __device__ __forceinline__ void matrix_tile(SrcPtrs src, DstPtrs dst, const Tab &tab, uint32_t k,
                                             bool ow, uint64_t col0, uint64_t ncols,
-                                            uint64_t tile, uint32_t lane, uint32_t mrows = M) {
+                                            uint64_t tile, uint32_t lane) {
     constexpr int P = D / 4;
     constexpr int G = D / B;
     static_assert(D % 4 == 0 && D % B == 0, "tile shape");
@@ -498,7 +491,7 @@ __device__ __forceinline__ void matrix_tile(SrcPtrs src, DstPtrs dst, const Tab 
 #pragma unroll
         for (int i = 0; i < P; ++i) {
             w[4 * i + 0] = w[4 * i + 1] = w[4 * i + 2] = w[4 * i + 3] = 0;
-            if (!ow && tl.ok[i] && (!GUARD || j < (int)mrows))
+            if (!ow && tl.ok[i])
                 load16(&w[4 * i], dst[j] + base + tl.lo[i]);
         }
 #pragma unroll
@@ -511,19 +504,18 @@ __device__ __forceinline__ void matrix_tile(SrcPtrs src, DstPtrs dst, const Tab 
     for (uint32_t t = 0;;) {
         if (t + 2 < k) load_source<D>(xb[0], src, t + 2, base, tl);
         if (t + 3 < k) load_source<D>(xb[1], src, t + 3, base, tl);
-        consume_step<B, M, D, GUARD, FP>(acc, xa, tab, t, k, mrows);
+        consume_step<B, M, D, FP>(acc, xa, tab, t, k);
         t += 2;
         if (t >= k) break;
         if (t + 2 < k) load_source<D>(xa[0], src, t + 2, base, tl);
         if (t + 3 < k) load_source<D>(xa[1], src, t + 3, base, tl);
-        consume_step<B, M, D, GUARD, FP>(acc, xb, tab, t, k, mrows);
+        consume_step<B, M, D, FP>(acc, xb, tab, t, k);
         t += 2;
         if (t >= k) break;
     }
 
 #pragma unroll
     for (int j = 0; j < M; ++j) {
-        if (GUARD && j >= (int)mrows) break;
         uint32_t w[D];
 #pragma unroll
         for (int g = 0; g < G; ++g) from_planes<B>(&acc[j][g * B], &w[g * B]);
@@ -596,7 +588,7 @@ __device__ __forceinline__ void matrix_run(SrcPtrs src, DstPtrs dst, const Tab &
         for (int r = 0; r < R; ++r) {
             if (cs >= nsteps) break;
             if (ct < k) {
-                consume<B, M, D, 1, false, FP>(acc, ring[r], ring[r], tab.at(ct), M);
+                consume<B, M, D, 1, FP>(acc, ring[r], ring[r], tab.at(ct));
             } else {
                 const uint32_t jd = ct - k;
                 uint32_t pl[D];
