@@ -283,8 +283,17 @@ def bench_configs(seed: int = 31) -> dict:
     run()
     want = codec.host_dot_product_matrix(Gf8D, vals, [s[:probe].cpu().numpy() for s in src], 4)
     assert np.array_equal(rows[:, :probe].cpu().numpy(), want)
+    # The same call straight through the C ABI with its pointer arrays built
+    # once: what a native (C, C++, Rust) caller pays per call, without the
+    # Python mirror's per-call argument marshalling.
+    import ctypes
+    from fgf_amd import _lib
+    ptrs, lens = ops._ptr_array(list(src))
+    sid = torch.cuda.current_stream().cuda_stream
+    native = lambda: _lib.lib().fgf_dot_product_matrix_with(
+        Gf8D.id, rows.data_ptr(), 4 * L, L, 4, plan._h, ptrs, lens, 10, sid)
     legs.append(_leg("Gf8D RS(10+4) encode, 1 MiB shards (cache-resident)", _median_launch_ms(run),
-                     14 * L))
+                     14 * L, c_abi_median_ms=round(_median_launch_ms(native), 4)))
     del src, rows
 
     # k = 64, m = 16, 4 MiB: scattered reconstruct of 16 erased slots.
