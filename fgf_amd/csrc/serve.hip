@@ -95,10 +95,11 @@ struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
     bool busy = false;
-    size_t cap_stripes = 0, cap_items = 0, cap_ptrs = 0;
-    fgf::StripeDesc *d_stripes = nullptr, *h_stripes = nullptr;
-    fgf::WorkItem *d_items = nullptr, *h_items = nullptr;
-    const void **d_ptrs = nullptr, **h_ptrs = nullptr;
+    // One descriptor block per launch, uploaded with one copy: the shard
+    // pointer table, the stripe descriptors and the work items, each at a
+    // 256-byte aligned offset (device memory, with pinned host staging).
+    size_t cap_blob = 0;
+    uint8_t *d_blob = nullptr, *h_blob = nullptr;
     uint32_t *d_counter = nullptr;
     std::vector<Req> reqs;
     std::vector<PlanEntry *> plans;
@@ -296,59 +297,82 @@ void finish_slot(fgf_server *s, Slot &sl, int status) {
     sl.busy = false;
 }
 
+constexpr size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// Bytes of a slot's descriptor block for the given counts.
+size_t blob_bytes(size_t nptrs, size_t nstripes, size_t nitems) {
+    return align256(nptrs * sizeof(void *)) + align256(nstripes * sizeof(StripeDesc)) +
+           nitems * sizeof(WorkItem);
+}
+
 // Builds and launches one slot's batch from the admitted requests.
 int launch_slot(fgf_server *s, Slot &sl) {
     const int f = sl.reqs.front().r.field;
     const int nb = field_bytes(f);
+    const size_t edw = entry_dwords(nb);
+    // Stripe geometry first (the work items need only lengths and counts),
+    // then the block is sized, then the device pointers are filled in.
     std::vector<StripeDesc> stripes(sl.reqs.size());
     size_t nptrs = 0;
-    for (const Req &q : sl.reqs) nptrs += q.r.k + q.r.m;
-    hipError_t e = grow(sl.d_ptrs, sl.h_ptrs, sl.cap_ptrs, nptrs);
-    if (e == hipSuccess) e = grow(sl.d_stripes, sl.h_stripes, sl.cap_stripes, stripes.size());
-    if (e != hipSuccess) return sdev("slot buffers", e);
-    size_t p = 0;
-    const size_t edw = entry_dwords(nb);
     for (size_t i = 0; i < sl.reqs.size(); ++i) {
         const Req &q = sl.reqs[i];
         const fgf_plan *plan = sl.plans[i]->plan;
         StripeDesc &d = stripes[i];
-        const size_t p0 = p;
-        if (!q.r.reconstruct) {
-            for (uint32_t t = 0; t < q.r.k; ++t) sl.h_ptrs[p++] = q.shards[t];
-            for (uint32_t j = 0; j < q.r.m; ++j) sl.h_ptrs[p++] = q.shards[q.r.k + j];
-        } else {
-            for (uint32_t t : survivors_of(q)) sl.h_ptrs[p++] = q.shards[t];
-            for (uint32_t e2 : q.erased) sl.h_ptrs[p++] = q.shards[e2];
-        }
-        d.srcs = (const uint8_t *const *)(sl.d_ptrs + p0);
-        d.rows = (uint8_t *const *)(sl.d_ptrs + p0 + plan->rows);
-        d.tab = plan->d_tab;
         d.row_len = q.r.row_len;
         d.k = (uint32_t)plan->rows;
         d.m = (uint32_t)plan->cols;
+        d.tab = plan->d_tab;
         d.tab_stride_t = (uint32_t)(plan->cols * edw);
         d.tab_stride_j = (uint32_t)edw;
         d.overwrite = 1;
         d.pad = 0;
+        nptrs += plan->rows + plan->cols;
     }
+    // Item size: a launch too small to give every resident wave one wave
+    // tile uses items of a single tile, spreading a lone request over as many
+    // waves as it has tiles (its latency is one wave's serial work); larger
+    // launches use the configured multi-tile items.
+    const uint64_t tile_bytes = 256u * (uint64_t)s->cfg.dwords;
+    uint64_t tiles = 0;
+    for (const StripeDesc &d : stripes)
+        tiles += (d.row_len + tile_bytes - 1) / tile_bytes *
+                 ((d.m + (uint32_t)s->cfg.rows_per_item - 1) / (uint32_t)s->cfg.rows_per_item);
+    const int subtiles =
+        tiles <= (uint64_t)s->cus * (uint64_t)s->cfg.waves_per_cu ? 1 : s->cfg.subtiles;
     std::vector<WorkItem> items =
-        build_work_items(stripes, s->cfg.dwords, s->cfg.rows_per_item, s->cfg.subtiles, 1);
-    if ((e = grow(sl.d_items, sl.h_items, sl.cap_items, items.size())) != hipSuccess)
-        return sdev("slot buffers", e);
-    std::memcpy(sl.h_stripes, stripes.data(), stripes.size() * sizeof(StripeDesc));
-    std::memcpy(sl.h_items, items.data(), items.size() * sizeof(WorkItem));
-    if ((e = hipMemcpyAsync(sl.d_ptrs, sl.h_ptrs, p * sizeof(void *), hipMemcpyHostToDevice, sl.stream)) ||
-        (e = hipMemcpyAsync(sl.d_stripes, sl.h_stripes, stripes.size() * sizeof(StripeDesc),
-                            hipMemcpyHostToDevice, sl.stream)) ||
-        (e = hipMemcpyAsync(sl.d_items, sl.h_items, items.size() * sizeof(WorkItem),
-                            hipMemcpyHostToDevice, sl.stream)))
+        build_work_items(stripes, s->cfg.dwords, s->cfg.rows_per_item, subtiles, 1);
+    const size_t off_str = align256(nptrs * sizeof(void *));
+    const size_t off_items = off_str + align256(stripes.size() * sizeof(StripeDesc));
+    const size_t total = blob_bytes(nptrs, stripes.size(), items.size());
+    hipError_t e = grow(sl.d_blob, sl.h_blob, sl.cap_blob, total);
+    if (e != hipSuccess) return sdev("slot buffers", e);
+    const void **h_ptrs = (const void **)sl.h_blob;
+    const void **d_ptrs = (const void **)sl.d_blob;
+    size_t p = 0;
+    for (size_t i = 0; i < sl.reqs.size(); ++i) {
+        const Req &q = sl.reqs[i];
+        StripeDesc &d = stripes[i];
+        const size_t p0 = p;
+        if (!q.r.reconstruct) {
+            for (uint32_t t = 0; t < q.r.k; ++t) h_ptrs[p++] = q.shards[t];
+            for (uint32_t j = 0; j < q.r.m; ++j) h_ptrs[p++] = q.shards[q.r.k + j];
+        } else {
+            for (uint32_t t : survivors_of(q)) h_ptrs[p++] = q.shards[t];
+            for (uint32_t e2 : q.erased) h_ptrs[p++] = q.shards[e2];
+        }
+        d.srcs = (const uint8_t *const *)(d_ptrs + p0);
+        d.rows = (uint8_t *const *)(d_ptrs + p0 + d.k);
+    }
+    std::memcpy(sl.h_blob + off_str, stripes.data(), stripes.size() * sizeof(StripeDesc));
+    std::memcpy(sl.h_blob + off_items, items.data(), items.size() * sizeof(WorkItem));
+    if ((e = hipMemcpyAsync(sl.d_blob, sl.h_blob, total, hipMemcpyHostToDevice, sl.stream)))
         return sdev("descriptor upload", e);
     const uint32_t wpb = batch_block_threads() / 64;
     const uint64_t want = (uint64_t)s->cus * s->cfg.waves_per_cu;
     const uint64_t waves = std::max<uint64_t>(1, std::min<uint64_t>(want, items.size()));
     const uint32_t blocks = (uint32_t)((waves + wpb - 1) / wpb);
-    BatchArgs a{sl.d_stripes, sl.d_items, (uint32_t)items.size(), blocks * wpb,
-                (uint32_t)s->cfg.subtiles, sl.d_counter};
+    BatchArgs a{(const StripeDesc *)(sl.d_blob + off_str), (const WorkItem *)(sl.d_blob + off_items),
+                (uint32_t)items.size(), blocks * wpb, (uint32_t)subtiles, sl.d_counter};
     if ((e = launch_batch(f, s->cfg.dwords, s->cfg.rows_per_item, a, blocks, sl.stream)) ||
         (e = hipEventRecord(sl.done, sl.stream)))
         return sdev("batch launch", e);
@@ -435,11 +459,16 @@ void scheduler(fgf_server *s) {
         }
         if (s->stop && !s->queued && !any_busy && !launched) break;
         if (!launched) {
-            // Idle or waiting on the device: poll completions at a fine grain.
-            if (any_busy)
-                s->cv_work.wait_for(lk, std::chrono::microseconds(20));
-            else
+            if (any_busy) {
+                // A launch is in flight: poll its completion event at once
+                // again (a timed wait would oversleep by tens of us). The
+                // scheduler thread spins while the device works.
+                lk.unlock();
+                std::this_thread::yield();
+                lk.lock();
+            } else {
                 s->cv_work.wait_for(lk, std::chrono::milliseconds(5));
+            }
         }
     }
     s->cv_done.notify_all();
@@ -466,20 +495,17 @@ int fgf_server_create(const fgf_server_config *cfg, fgf_server **out) {
     (void)hipGetDevice(&s->device);
     (void)hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, s->device);
     s->slots.resize(cfg->slots);
-    // Descriptor buffers sized for a full batch up front: growing them later
+    // Descriptor blocks sized for a full batch up front: growing one later
     // frees memory, and hipFree waits for every stream of the device.
-    const size_t want_stripes = cfg->max_batch_stripes;
-    const size_t want_ptrs = (size_t)cfg->max_batch_stripes * 32;
-    const size_t want_items = (size_t)cfg->max_batch_stripes * 128;
+    const size_t want = blob_bytes((size_t)cfg->max_batch_stripes * 32, cfg->max_batch_stripes,
+                                   (size_t)cfg->max_batch_stripes * 128);
     for (Slot &sl : s->slots) {
         hipError_t e;
         if ((e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking)) ||
             (e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming)) ||
             (e = hipMalloc((void **)&sl.d_counter, 2 * sizeof(uint32_t))) ||
             (e = hipMemset(sl.d_counter, 0, 2 * sizeof(uint32_t))) ||
-            (e = grow(sl.d_stripes, sl.h_stripes, sl.cap_stripes, want_stripes)) ||
-            (e = grow(sl.d_ptrs, sl.h_ptrs, sl.cap_ptrs, want_ptrs)) ||
-            (e = grow(sl.d_items, sl.h_items, sl.cap_items, want_items))) {
+            (e = grow(sl.d_blob, sl.h_blob, sl.cap_blob, want))) {
             fgf_server_destroy(s);
             return sdev("fgf_server_create", e);
         }
@@ -647,12 +673,8 @@ void fgf_server_destroy(fgf_server *s) {
     }
     for (Slot &sl : s->slots) {
         if (sl.stream) (void)hipStreamSynchronize(sl.stream);
-        if (sl.d_stripes) (void)hipFree(sl.d_stripes);
-        if (sl.h_stripes) (void)hipHostFree(sl.h_stripes);
-        if (sl.d_items) (void)hipFree(sl.d_items);
-        if (sl.h_items) (void)hipHostFree(sl.h_items);
-        if (sl.d_ptrs) (void)hipFree(sl.d_ptrs);
-        if (sl.h_ptrs) (void)hipHostFree(sl.h_ptrs);
+        if (sl.d_blob) (void)hipFree(sl.d_blob);
+        if (sl.h_blob) (void)hipHostFree(sl.h_blob);
         if (sl.d_counter) (void)hipFree(sl.d_counter);
         if (sl.done) (void)hipEventDestroy(sl.done);
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
