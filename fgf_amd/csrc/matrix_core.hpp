@@ -271,6 +271,34 @@ __device__ __forceinline__ void store16(uint8_t *p, const uint32_t *w) {
     __builtin_nontemporal_store(v, (g_u32x4 *)p);
 }
 
+// Raw buffer accesses over one tile of one stream: the hardware range check
+// returns zeros for a load and drops a store at an offset past the
+// resource's byte count, so a partial tile, or a step past the end of the
+// work, issues the same loads as a whole one. With no branch around any load
+// the compiler sees the same number of loads on every path and waits for
+// each ring slot with vmcnt(slots behind it) instead of draining the ring
+// with vmcnt(0) at every join. Offsets and byte counts are multiples of 16,
+// so a 16-byte access is either wholly in range or wholly out.
+typedef __amdgpu_buffer_rsrc_t BufRes;
+// Dword 3 of a raw buffer resource on gfx9-family parts: 32-bit data format
+// (a zero format would fail the range check of every access).
+constexpr int BUF_RES_DWORD3 = 0x00020000;
+constexpr int BUF_NT = 2;  // cache-policy operand: non-temporal
+
+__device__ __forceinline__ BufRes buf_res(const void *p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, BUF_RES_DWORD3);
+}
+
+__device__ __forceinline__ void bload16(uint32_t *x, BufRes r, uint32_t off) {
+    const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, BUF_NT);
+    x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
+}
+
+__device__ __forceinline__ void bstore16(BufRes r, uint32_t off, const uint32_t *w) {
+    const u32x4 v = {w[0], w[1], w[2], w[3]};
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, off, 0, BUF_NT);
+}
+
 
 template <int D, typename SrcPtrs>
 __device__ __forceinline__ void load_source(uint32_t (&x)[D], SrcPtrs src, uint32_t t, uint64_t base,
@@ -553,18 +581,23 @@ __device__ __forceinline__ void matrix_run(SrcPtrs src, DstPtrs dst, const Tab &
 #pragma unroll
     for (int i = 0; i < P; ++i) lo[i] = (uint32_t)i * 1024u + lane * 16u;
 
-    // Issue cursor: step, tile, step within the tile.
+    // Bytes before ncols of tile t0 + i, in 32-bit wave-uniform arithmetic:
+    // whole tiles, then the tail tile, then nothing.
+    const uint32_t t0 = (uint32_t)(tile0 / CHUNK), nfull = (uint32_t)(ncols / CHUNK);
+    const uint32_t tail = (uint32_t)(ncols % CHUNK);
+    auto tile_bytes = [&](uint32_t i) -> uint32_t {
+        return t0 + i < nfull ? (uint32_t)CHUNK : t0 + i == nfull ? tail : 0u;
+    };
+
+    // Issue cursor: step, tile, step within the tile. A step past the end
+    // loads through an empty resource (zeros, no memory access).
     uint32_t is = 0, ic = 0, it = 0;
     auto issue = [&](uint32_t (&x)[D]) {
-        if (is < nsteps) {
-            const uint64_t base = tile0 + (uint64_t)ic * CHUNK;
-            const uint8_t *p = (it < k ? (const uint8_t *)src[it] : (const uint8_t *)dst[it - k]) + col0;
+        const uint64_t base = tile0 + (uint64_t)ic * CHUNK;
+        const uint8_t *p = (it < k ? (const uint8_t *)src[it] : (const uint8_t *)dst[it - k]) + col0 + base;
+        const BufRes r = buf_res(p, is < nsteps ? tile_bytes(ic) : 0u);
 #pragma unroll
-            for (int i = 0; i < P; ++i) {
-                x[4 * i + 0] = x[4 * i + 1] = x[4 * i + 2] = x[4 * i + 3] = 0;
-                if (base + lo[i] < ncols) load16(&x[4 * i], p + base + lo[i]);
-            }
-        }
+        for (int i = 0; i < P; ++i) bload16(&x[4 * i], r, lo[i]);
         ++is;
         if (++it == per) {
             it = 0;
@@ -572,9 +605,15 @@ __device__ __forceinline__ void matrix_run(SrcPtrs src, DstPtrs dst, const Tab &
         }
     };
 
+    // The prologue's loads stay in slot order: the wait for slot r counts
+    // the loads issued after it, and a reordered prologue would leave slot 0
+    // with none behind it.
     uint32_t ring[R][D];
 #pragma unroll
-    for (int r = 0; r < R; ++r) issue(ring[r]);
+    for (int r = 0; r < R; ++r) {
+        issue(ring[r]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
 
     uint32_t acc[M][D];
 #pragma unroll
@@ -582,12 +621,18 @@ __device__ __forceinline__ void matrix_run(SrcPtrs src, DstPtrs dst, const Tab &
 #pragma unroll
         for (int d = 0; d < D; ++d) acc[j][d] = 0;
 
-    uint32_t cs = 0, cc = 0, ct = 0;  // consume cursor
-    while (cs < nsteps) {
+    // Consume cursor. The loop runs whole passes over the ring and leaves
+    // only after slot R - 1; steps past the end skip their fold but still
+    // issue their (empty) load. With no exit in the middle of a pass, every
+    // path to slot r's fold has the same loads behind it, and the wait
+    // for the slot counts them instead of draining the ring.
+    uint32_t cs = 0, cc = 0, ct = 0;
+    do {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            if (cs >= nsteps) break;
-            if (ct < k) {
+            const bool real = cs < nsteps;
+            if (!real) {
+            } else if (ct < k) {
                 consume<B, M, D, 1, FP>(acc, ring[r], ring[r], tab.at(ct));
             } else {
                 const uint32_t jd = ct - k;
@@ -602,16 +647,17 @@ __device__ __forceinline__ void matrix_run(SrcPtrs src, DstPtrs dst, const Tab &
             }
             issue(ring[r]);
             ++cs;
-            if (++ct == per) {
+            if (real && ++ct == per) {
                 const uint64_t base = tile0 + (uint64_t)cc * CHUNK;
+                const uint32_t n = tile_bytes(cc);
 #pragma unroll
                 for (int j = 0; j < M; ++j) {
                     uint32_t w[D];
 #pragma unroll
                     for (int g = 0; g < G; ++g) from_planes<B>(&acc[j][g * B], &w[g * B]);
+                    const BufRes r = buf_res((uint8_t *)dst[j] + col0 + base, n);
 #pragma unroll
-                    for (int i = 0; i < P; ++i)
-                        if (base + lo[i] < ncols) store16((uint8_t *)dst[j] + col0 + base + lo[i], &w[4 * i]);
+                    for (int i = 0; i < P; ++i) bstore16(r, lo[i], &w[4 * i]);
 #pragma unroll
                     for (int d = 0; d < D; ++d) acc[j][d] = 0;
                 }
@@ -619,7 +665,7 @@ __device__ __forceinline__ void matrix_run(SrcPtrs src, DstPtrs dst, const Tab &
                 ++cc;
             }
         }
-    }
+    } while (cs < nsteps);
 }
 
 }  // namespace core

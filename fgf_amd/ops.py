@@ -96,10 +96,13 @@ class Plan:
                                           len(coeffs), ctypes.byref(h)))
         return cls(field, h.value)
 
-    def __del__(self):
+    # The loader module is bound as a default: at interpreter exit this
+    # module's globals may already be cleared when the last handles go.
+    def __del__(self, _mod=_lib):
         h = getattr(self, "_h", None)
-        if h is not None and h.value and _lib._lib is not None:
-            _lib.lib().fgf_plan_free(h)
+        lib = getattr(_mod, "_lib", None)
+        if h is not None and h.value and lib is not None:
+            lib.fgf_plan_free(h)
             self._h = None
 
     def __len__(self) -> int:
