@@ -60,8 +60,17 @@ __device__ __forceinline__ void item_by_rows(uint32_t mrows, const StripeDesc &s
      ...);
 }
 
+// Waves per SIMD the register allocator is held to, where one more wave
+// fits by trimming a few registers and the trim spills nothing inside the
+// pass loop (8 rows x 8 dwords of GF(2^8): 172 -> 168 VGPRs, 2 -> 3 waves;
+// GF(2^32) 4 rows x 8 dwords: 130 -> 128, 3 -> 4 waves).
+constexpr int batch_waves_per_eu(int nb, int m, int d) {
+    return nb == 1 && m == 8 && d == 8 ? 3 : nb == 4 && m == 4 && d == 8 ? 4 : 1;
+}
+
 template <int B, bool FP, int M, int D>
-__global__ __launch_bounds__(BATCH_BLOCK) void k_batch(const BatchArgs a) {
+__global__ __launch_bounds__(BATCH_BLOCK) __attribute__((amdgpu_waves_per_eu(batch_waves_per_eu(B, M, D))))
+void k_batch(const BatchArgs a) {
     const uint32_t lane = threadIdx.x & 63u;
     constexpr uint64_t CHUNK = 256u * D;
     // The next item is claimed before the current one is processed: the
