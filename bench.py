@@ -182,7 +182,7 @@ def bench_batched(n_stripes: int = 4096, seed: int = 4096, reps: int = 5) -> dic
 
 
 def profile_batched(launches: int, n_stripes: int = 4096, seed: int = 4096,
-                    config=(4, 8, 4, 32, 1)) -> dict:
+                    config=(8, 8, 4, 12, 1)) -> dict:
     """Only the batched launch of the uneven-stripe mix, `launches` times, at
     a fixed tile config (dwords, rows per item, subtiles, waves per CU,
     order): the run that the batch kernel's rocprofv3 counter and trace
@@ -401,6 +401,7 @@ def bench_serving(n_requests: int = 4096, seed: int = 77,
     reqs = [workloads.serve_request(pool, sp) for sp in specs]
     total = sum((sp.k + len(sp.out_shards)) * sp.row_len for sp in specs)
     srv = Server()
+    server_cfg = dict(srv.config)
     srv.replay(reqs[:256], [0.0] * 256)             # warm: plans, slot buffers, code objects
     legs = []
     rng = np.random.default_rng(seed + 2)
@@ -481,8 +482,7 @@ def bench_serving(n_requests: int = 4096, seed: int = 77,
         "workload": f"{n_requests} requests, 80% Gf8D / 20% Gf16, 256 coefficient matrices, "
                     f"k~U[4,32], m~U[2,8], L log-uniform 4KiB-1MiB, 50% reconstruct matrices",
         "algorithmic_bytes": total,
-        "server": {"slots": 4, "hbm_budget_GiB": 64, "max_batch_stripes": 512,
-                   "tile": {"dwords": 4, "rows_per_item": 8, "subtiles": 4, "waves_per_cu": 8}},
+        "server": server_cfg,
         "legs": legs,
         "per_call": {"ms": round(t_call, 3), "GBps": round(per_call_gbps, 1)},
         "burst_speedup_vs_per_call": round(burst[0]["GBps"] / per_call_gbps, 2) if burst else None,

@@ -202,7 +202,7 @@ struct fgf_batch {
     int order = 1;
     int rows_per_item = 8;
     int subtiles = 4;
-    uint32_t waves_per_cu = 16;
+    uint32_t waves_per_cu = 12;
     size_t nstripes = 0;
     std::vector<fgf::StripeDesc> stripes;   // host copy (device pointers inside)
     fgf::StripeDesc *d_stripes = nullptr;
@@ -310,7 +310,7 @@ int fgf_batch_create(fgf_field f, const fgf_stripe *stripes, size_t n, fgf_batch
     if (!b) return bfail("out of host memory");
     b->field = f;
     b->nb = nb;
-    b->dwords = fgf::batch_min_dwords(nb) == 4 ? 4 : 8;
+    b->dwords = 8;
     b->rows_per_item = fgf::batch_default_rows(nb);
     b->nstripes = n;
     std::vector<const void *> ptrs;
