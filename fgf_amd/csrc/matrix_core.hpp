@@ -489,67 +489,77 @@ __device__ __forceinline__ void consume_step(uint32_t (&acc)[M][D], const uint32
 }
 
 // One wave's tile of rows[j] (=|^=) sum_t c(t, j) * src[t], for columns
-// [col0 + tile, col0 + tile + 256*D) clipped to col0 + ncols. All pointers
-// + col0 must be 16-byte aligned and ncols a multiple of 16. Exactly M rows
-// are processed (a compile-time count keeps every table load hoistable out
-// of the dword loop). `ow` selects overwrite (zero seed) or accumulate
-// (seed from the destination). Sources are consumed in pairs from two
-// ping-pong register buffers: the next pair's loads are in flight while the
-// current pair is multiplied.
+// [col0 + tile, col0 + tile + 256*D) clipped to col0 + ncols (tile <
+// ncols). All pointers + col0 must be 16-byte aligned and ncols a multiple
+// of 16. Exactly M rows are processed (a compile-time count keeps every
+// table load hoistable out of the dword loop). `ow` selects overwrite (zero
+// seed) or accumulate (seed from the destination). Sources are consumed in
+// pairs from two ping-pong register buffers: the next pair's loads are in
+// flight while the current pair is multiplied. Streams go through raw
+// buffer resources sized to the tile's bytes, so a ragged tile needs no
+// per-lane branch, and a source index past k loads zeros from an empty
+// resource; every pass of the loop issues the same four loads and leaves
+// only at its end, so each pair waits for its own loads alone.
 template <int B, int M, int D, bool FP = false, typename SrcPtrs, typename DstPtrs, typename Tab>
 __device__ __forceinline__ void matrix_tile(SrcPtrs src, DstPtrs dst, const Tab &tab, uint32_t k,
                                             bool ow, uint64_t col0, uint64_t ncols,
                                             uint64_t tile, uint32_t lane) {
     constexpr int P = D / 4;
     constexpr int G = D / B;
+    constexpr uint32_t CHUNK = 256u * D;
     static_assert(D % 4 == 0 && D % B == 0, "tile shape");
 
-    TileLanes<P> tl;
+    uint32_t lo[P];
 #pragma unroll
-    for (int i = 0; i < P; ++i) {
-        tl.lo[i] = (uint32_t)i * 1024u + lane * 16u;
-        tl.ok[i] = tile + tl.lo[i] < ncols;
-    }
+    for (int i = 0; i < P; ++i) lo[i] = (uint32_t)i * 1024u + lane * 16u;
+    const uint32_t nb = ncols - tile < CHUNK ? (uint32_t)(ncols - tile) : CHUNK;
     const uint64_t base = col0 + tile;
+
+    auto load_src = [&](uint32_t (&x)[D], uint32_t t) {
+        const bool real = t < k;
+        const BufRes r = buf_res((const uint8_t *)src[real ? t : 0u] + base, real ? nb : 0u);
+#pragma unroll
+        for (int i = 0; i < P; ++i) bload16(&x[4 * i], r, lo[i]);
+    };
 
     uint32_t acc[M][D];
 #pragma unroll
     for (int j = 0; j < M; ++j) {
         uint32_t w[D];
+        if (ow) {
 #pragma unroll
-        for (int i = 0; i < P; ++i) {
-            w[4 * i + 0] = w[4 * i + 1] = w[4 * i + 2] = w[4 * i + 3] = 0;
-            if (!ow && tl.ok[i])
-                load16(&w[4 * i], dst[j] + base + tl.lo[i]);
+            for (int d = 0; d < D; ++d) w[d] = 0;
+        } else {
+            const BufRes r = buf_res((const uint8_t *)dst[j] + base, nb);
+#pragma unroll
+            for (int i = 0; i < P; ++i) bload16(&w[4 * i], r, lo[i]);
         }
 #pragma unroll
         for (int g = 0; g < G; ++g) to_planes<B>(&w[g * B], &acc[j][g * B]);
     }
 
     uint32_t xa[2][D], xb[2][D];
-    load_source<D>(xa[0], src, 0, base, tl);
-    if (k > 1) load_source<D>(xa[1], src, 1, base, tl);
-    for (uint32_t t = 0;;) {
-        if (t + 2 < k) load_source<D>(xb[0], src, t + 2, base, tl);
-        if (t + 3 < k) load_source<D>(xb[1], src, t + 3, base, tl);
+    load_src(xa[0], 0);
+    load_src(xa[1], 1);
+    uint32_t t = 0;
+    do {
+        load_src(xb[0], t + 2);
+        load_src(xb[1], t + 3);
         consume_step<B, M, D, FP>(acc, xa, tab, t, k);
-        t += 2;
-        if (t >= k) break;
-        if (t + 2 < k) load_source<D>(xa[0], src, t + 2, base, tl);
-        if (t + 3 < k) load_source<D>(xa[1], src, t + 3, base, tl);
-        consume_step<B, M, D, FP>(acc, xb, tab, t, k);
-        t += 2;
-        if (t >= k) break;
-    }
+        load_src(xa[0], t + 4);
+        load_src(xa[1], t + 5);
+        if (t + 2 < k) consume_step<B, M, D, FP>(acc, xb, tab, t + 2, k);
+        t += 4;
+    } while (t < k);
 
 #pragma unroll
     for (int j = 0; j < M; ++j) {
         uint32_t w[D];
 #pragma unroll
         for (int g = 0; g < G; ++g) from_planes<B>(&acc[j][g * B], &w[g * B]);
+        const BufRes r = buf_res((uint8_t *)dst[j] + base, nb);
 #pragma unroll
-        for (int i = 0; i < P; ++i)
-            if (tl.ok[i]) store16(dst[j] + base + tl.lo[i], &w[4 * i]);
+        for (int i = 0; i < P; ++i) bstore16(r, lo[i], &w[4 * i]);
     }
 }
 
