@@ -102,9 +102,15 @@ __device__ __forceinline__ bool map_block(const MatrixArgs &a, uint32_t &ct, uin
 // most LDS_TABLE_BYTES), then every wave runs its tile.
 // GF(2^64) asks for three waves per SIMD: its 81 Karatsuba selector
 // registers would otherwise leave two, too few to cover its 27 table reads
-// per source and row (measured 11% faster at two rows per block).
+// per source and row (measured 11% faster at two rows per block). GF(2^16)
+// at 7-8 rows and accumulating GF(2^8) at 13-16 rows sit a few registers
+// above the three-wave limit of 168 VGPRs and are held to it as well.
+constexpr int matrix_waves_per_eu(int nb, int m, bool ow) {
+    return nb == 8 || (nb == 2 && m >= 7) || (nb == 1 && m >= 13 && !ow) ? 3 : 1;
+}
+
 template <int B, bool FP, int M, int D, bool OW>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(B == 8 ? 3 : 1)))
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(matrix_waves_per_eu(B, M, OW))))
 void k_matrix(const MatrixArgs a) {
     uint32_t ct, rg;
     if (!map_block(a, ct, rg)) return;
