@@ -47,9 +47,14 @@ __device__ __forceinline__ void item_tiles(const StripeDesc &s, uint32_t j0, uin
     const core::GlobalTab<B> tab{s.tab + (size_t)j0 * s.tab_stride_j, s.tab_stride_t, s.tab_stride_j};
     const uint64_t left = (s.row_len - tile + CHUNK - 1) / CHUNK;
     const uint32_t nsub = (uint32_t)min<uint64_t>(subtiles, left);
-    core::matrix_run<B, MC, D, FP, ring_slots(B)>(core::ConstPtrs<const uint8_t>{s.srcs},
-                                                  core::ConstPtrs<uint8_t>{s.rows + j0}, tab, s.k,
-                                                  s.overwrite != 0, 0, s.row_len, tile, nsub, lane);
+    const core::ConstPtrs<const uint8_t> src{s.srcs};
+    const core::ConstPtrs<uint8_t> dst{s.rows + j0};
+    if (s.overwrite)
+        core::matrix_run<B, MC, D, FP, ring_slots(B), false>(src, dst, tab, s.k, 0, s.row_len, tile,
+                                                             nsub, lane);
+    else
+        core::matrix_run<B, MC, D, FP, ring_slots(B), true>(src, dst, tab, s.k, 0, s.row_len, tile,
+                                                            nsub, lane);
 }
 
 template <int B, bool FP, int D, int... Ms>

@@ -576,15 +576,15 @@ __device__ __forceinline__ void matrix_tile(SrcPtrs src, DstPtrs dst, const Tab 
 // cleared. Columns are relative to col0; those at or past ncols are neither
 // loaded nor stored. Every pointer + col0 must be 16-byte aligned and ncols
 // a multiple of 16.
-template <int B, int M, int D, bool FP, int R, typename SrcPtrs, typename DstPtrs, typename Tab>
+template <int B, int M, int D, bool FP, int R, bool ACC, typename SrcPtrs, typename DstPtrs, typename Tab>
 __device__ __forceinline__ void matrix_run(SrcPtrs src, DstPtrs dst, const Tab &tab, uint32_t k,
-                                           bool ow, uint64_t col0, uint64_t ncols, uint64_t tile0,
+                                           uint64_t col0, uint64_t ncols, uint64_t tile0,
                                            uint32_t nsub, uint32_t lane) {
     constexpr int P = D / 4;
     constexpr int G = D / B;
     constexpr uint64_t CHUNK = 256u * D;
     static_assert(D % 4 == 0 && D % B == 0 && R >= 2, "run shape");
-    const uint32_t per = k + (ow ? 0u : (uint32_t)M);
+    const uint32_t per = k + (ACC ? (uint32_t)M : 0u);
     const uint32_t nsteps = nsub * per;
 
     uint32_t lo[P];
@@ -641,7 +641,13 @@ __device__ __forceinline__ void matrix_run(SrcPtrs src, DstPtrs dst, const Tab &
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const bool real = cs < nsteps;
-            if (!real) {
+            if constexpr (!ACC) {
+                // Every step is a source. A step past the end folds the
+                // zeros of its empty load (any table maps 0 to 0), so the
+                // fold needs no branch and the accumulators stay in place
+                // instead of being copied back at a join after every step.
+                consume<B, M, D, 1, FP>(acc, ring[r], ring[r], tab.at(ct));
+            } else if (!real) {
             } else if (ct < k) {
                 consume<B, M, D, 1, FP>(acc, ring[r], ring[r], tab.at(ct));
             } else {
