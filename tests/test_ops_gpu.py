@@ -205,6 +205,33 @@ def test_dot_product_matrix_overwrites(f):
                     assert np.array_equal(host(prepared), want)
 
 
+@pytest.mark.parametrize("f", [Gf8D, Gf16, Gf64, FanPaar32], ids=lambda f: NAMES[f.id])
+def test_matrix_every_source_count_mod_4_over_ragged_tiles(f):
+    # The vector kernel folds sources in pairs, two pairs per pass of its
+    # loop, and loads a source index past k as zeros: k = 1..10 covers every
+    # k mod 4 and both halves of a pass ending early. Row lengths put the
+    # last wave tile at one 16-byte lane, mid-tile and a full tile, in both
+    # overwrite and accumulate mode.
+    name = NAMES[f.id]
+    tile = 256 * (4 if f.BYTES == 1 else 8)                # bytes per wave tile
+    nrows = 3
+    for row_len in (tile + 16, 3 * tile + tile // 2, 4 * tile):   # bytes
+        for k in range(1, 11):
+            srcs = [noise(row_len, 0x900 + 17 * k + t) for t in range(k)]
+            coeffs = rand_coeffs(f, k * nrows, 0xA00 + k)
+            plan = ops.Plan.matrix(f, k, nrows, coeffs)
+            d_srcs = [dev(x) for x in srcs]
+            got = dev(noise(row_len * nrows, 0xB00 + k))
+            ops.dot_product_matrix_with(f, got, row_len, nrows, plan, d_srcs)
+            want = oracle_matrix(name, row_len, nrows, coeffs, srcs)
+            assert np.array_equal(host(got), want), (row_len, k, "overwrite")
+            init = noise(row_len * nrows, 0xC00 + k)
+            acc = dev(init)
+            ops.mul_add_matrix_with(f, acc, row_len, nrows, plan, d_srcs)
+            want = oracle_matrix(name, row_len, nrows, coeffs, srcs, init)
+            assert np.array_equal(host(acc), want), (row_len, k, "accumulate")
+
+
 @pytest.mark.parametrize("f", ALL_FIELDS, ids=lambda f: NAMES[f.id])
 def test_matrix_scattered_matches_contiguous(f):
     # tests/ops.rs:406-478: rows separated by element-aligned gaps of 7
