@@ -231,48 +231,70 @@ __device__ __forceinline__ void build_product_table(uint8_t *T, uint8_t *s_ex, u
 }
 
 // Aligned body: every pointer 16-byte aligned, nbytes a multiple of 16.
-// Each wave step covers EW_TILE bytes; pieces at or past nbytes are neither
-// loaded nor stored.
+// Each wave step covers EW_TILE bytes as two 1 KiB pieces per operand. The
+// next step's operands are loaded while the current step is multiplied, in
+// two ping-pong register sets. Operands and products go through raw buffer
+// resources sized to the step's bytes before nbytes: a ragged last step
+// needs no per-lane branch, a step past the end loads zeros from an empty
+// resource, and every pass of the loop issues the same loads and leaves only
+// at its end, so each step waits for its own loads alone.
+template <int F>
+__device__ __forceinline__ void ew_step(uint8_t *dst, const uint32_t (&xa)[8], const uint32_t (&xb)[8],
+                                        uint64_t tile, uint32_t n, uint32_t lane, const uint8_t *s_prod) {
+    constexpr int NB = field_bytes(F);
+    constexpr bool FP = is_fan_paar(F);
+    uint32_t r[8];
+    if constexpr (NB == 1) {
+#pragma unroll
+        for (int d = 0; d < 8; ++d) r[d] = byte_products(xa[d], xb[d], s_prod);
+    } else {
+#pragma unroll
+        for (int g = 0; g < 8 / NB; ++g) {
+            uint32_t pa[NB], pb[NB], pr[NB];
+            to_planes<NB>(&xa[g * NB], pa);
+            to_planes<NB>(&xb[g * NB], pb);
+            mul_planes<NB, FP>(pa, pb, pr, s_prod);
+            from_planes<NB>(pr, &r[g * NB]);
+        }
+    }
+    const BufRes rd = buf_res(dst + tile, n);
+    bstore16(rd, lane * 16u, &r[0]);
+    bstore16(rd, lane * 16u + 1024u, &r[4]);
+}
+
 template <int F>
 __global__ __launch_bounds__(EW_BLOCK) void k_elementwise(uint8_t *dst, const uint8_t *a,
                                                            const uint8_t *b, uint64_t nbytes) {
     __shared__ __attribute__((aligned(16))) uint8_t s_prod[65536];
     __shared__ uint8_t s_ex[512], s_lg[256];
     build_product_table<F>(s_prod, s_ex, s_lg);
-    constexpr int NB = field_bytes(F);
-    constexpr bool FP = is_fan_paar(F);
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint64_t stride = (uint64_t)gridDim.x * (EW_BLOCK / 64) * EW_TILE;
-    for (uint64_t tile = ((uint64_t)blockIdx.x * (EW_BLOCK / 64) + wave) * EW_TILE; tile < nbytes;
-         tile += stride) {
-        const uint64_t o0 = tile + lane * 16u, o1 = o0 + 1024u;
-        const bool ok0 = o0 < nbytes, ok1 = o1 < nbytes;
-        uint32_t xa[8] = {0, 0, 0, 0, 0, 0, 0, 0}, xb[8] = {0, 0, 0, 0, 0, 0, 0, 0}, r[8];
-        if (ok0) {
-            load16(&xa[0], a + o0);
-            load16(&xb[0], b + o0);
-        }
-        if (ok1) {
-            load16(&xa[4], a + o1);
-            load16(&xb[4], b + o1);
-        }
-        if constexpr (NB == 1) {
-#pragma unroll
-            for (int d = 0; d < 8; ++d) r[d] = byte_products(xa[d], xb[d], s_prod);
-        } else {
-#pragma unroll
-            for (int g = 0; g < 8 / NB; ++g) {
-                uint32_t pa[NB], pb[NB], pr[NB];
-                to_planes<NB>(&xa[g * NB], pa);
-                to_planes<NB>(&xb[g * NB], pb);
-                mul_planes<NB, FP>(pa, pb, pr, s_prod);
-                from_planes<NB>(pr, &r[g * NB]);
-            }
-        }
-        if (ok0) store16(dst + o0, &r[0]);
-        if (ok1) store16(dst + o1, &r[4]);
-    }
+    uint64_t tile = ((uint64_t)blockIdx.x * (EW_BLOCK / 64) + wave) * EW_TILE;
+    if (tile >= nbytes) return;
+    auto bytes_at = [&](uint64_t t) -> uint32_t {
+        return t < nbytes ? (uint32_t)(nbytes - t < EW_TILE ? nbytes - t : EW_TILE) : 0u;
+    };
+    auto load = [&](uint32_t (&xa)[8], uint32_t (&xb)[8], uint64_t t) {
+        const uint32_t n = bytes_at(t);
+        const uint64_t o = n ? t : 0u;
+        const BufRes ra = buf_res(a + o, n), rb = buf_res(b + o, n);
+        bload16(&xa[0], ra, lane * 16u);
+        bload16(&xa[4], ra, lane * 16u + 1024u);
+        bload16(&xb[0], rb, lane * 16u);
+        bload16(&xb[4], rb, lane * 16u + 1024u);
+    };
+    uint32_t ca[8], cb[8], na[8], nb[8];
+    load(ca, cb, tile);
+    do {
+        load(na, nb, tile + stride);
+        ew_step<F>(dst, ca, cb, tile, bytes_at(tile), lane, s_prod);
+        load(ca, cb, tile + 2 * stride);
+        if (tile + stride < nbytes)
+            ew_step<F>(dst, na, nb, tile + stride, bytes_at(tile + stride), lane, s_prod);
+        tile += 2 * stride;
+    } while (tile < nbytes);
 }
 
 // One element per thread, byte loads, products through log/exp tables.

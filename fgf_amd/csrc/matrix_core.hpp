@@ -244,12 +244,7 @@ __device__ __forceinline__ void kara_accumulate(uint32_t *acc, const uint32_t *l
 // 16-byte pieces at base + lo[i], lo[i] = i*1024 + lane*16, so each load
 // instruction covers 1 KiB contiguous. `base` (= col0 + tile) is uniform;
 // the lane offsets are the same for every source and destination row, so
-// every access is an SGPR base plus a fixed 32-bit VGPR offset.
-template <int P>
-struct TileLanes {
-    uint32_t lo[P];
-    bool ok[P];
-};
+// every access is a scalar base plus a fixed 32-bit VGPR offset.
 
 // Streams are touched once, so loads and stores are non-temporal: on gfx950
 // this lets the 72/28 read/write mix of an encode run at about 6 TB/s where
@@ -299,17 +294,6 @@ __device__ __forceinline__ void bstore16(BufRes r, uint32_t off, const uint32_t 
     __builtin_amdgcn_raw_buffer_store_b128(v, r, off, 0, BUF_NT);
 }
 
-
-template <int D, typename SrcPtrs>
-__device__ __forceinline__ void load_source(uint32_t (&x)[D], SrcPtrs src, uint32_t t, uint64_t base,
-                                            const TileLanes<D / 4> &tl) {
-    const uint8_t *s = src[t] + base;
-#pragma unroll
-    for (int i = 0; i < D / 4; ++i) {
-        x[4 * i + 0] = x[4 * i + 1] = x[4 * i + 2] = x[4 * i + 3] = 0;
-        if (tl.ok[i]) load16(&x[4 * i], s + tl.lo[i]);
-    }
-}
 
 // Coefficient-table views. An entry is the five dwords of one byte constant
 // (T0 lo/hi, T1 lo/hi, T2); byte constant i of source t + s, row j is at
