@@ -43,7 +43,21 @@ def _buf(x) -> tuple[int, int]:
         return x.ptr, x.nbytes
     if not x.is_contiguous():
         raise Panic("buffer must be a contiguous byte range")
-    return x.data_ptr(), x.numel() * x.element_size()
+    return x.data_ptr(), x.nbytes
+
+
+_raw_stream = None
+
+
+def _current_stream(device_index: int) -> int:
+    """The current HIP stream of a device as an integer handle: torch's raw
+    accessor where it has one (a single call), else the Stream object."""
+    global _raw_stream
+    if _raw_stream is None:
+        import torch
+        _raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None) or (
+            lambda i: torch.cuda.current_stream(i).cuda_stream)
+    return int(_raw_stream(device_index))
 
 
 def _stream(stream, *bufs) -> int:
@@ -51,8 +65,7 @@ def _stream(stream, *bufs) -> int:
         return int(stream)
     for b in bufs:
         if not isinstance(b, Buf) and getattr(b, "is_cuda", False):
-            import torch
-            return torch.cuda.current_stream(b.device).cuda_stream
+            return _current_stream(b.get_device())
     return 0
 
 
@@ -62,10 +75,10 @@ def _coeff_bytes(field: Field, coeffs: Sequence[int]) -> ctypes.Array:
 
 
 def _ptr_array(bufs: Sequence) -> tuple[ctypes.Array, ctypes.Array]:
-    ptrs = (ctypes.c_void_p * max(len(bufs), 1))()
-    lens = (ctypes.c_size_t * max(len(bufs), 1))()
-    for i, b in enumerate(bufs):
-        ptrs[i], lens[i] = _buf(b)
+    n = max(len(bufs), 1)
+    pl = [_buf(b) for b in bufs]
+    ptrs = (ctypes.c_void_p * n)(*[p for p, _ in pl])
+    lens = (ctypes.c_size_t * n)(*[ln for _, ln in pl])
     return ptrs, lens
 
 
