@@ -305,11 +305,21 @@ size_t blob_bytes(size_t nptrs, size_t nstripes, size_t nitems) {
            nitems * sizeof(WorkItem);
 }
 
+// Kernel shape of a launch for one field: the configured dwords and rows
+// per item where the element width allows them, else the width's own
+// (4- and 8-byte towers take 8 dwords and 4 or 2 rows per item).
+void tile_for(const fgf_server *s, int nb, int &dwords, int &rows) {
+    dwords = std::max(s->cfg.dwords, batch_min_dwords(nb));
+    rows = batch_rows_allowed(nb, s->cfg.rows_per_item) ? s->cfg.rows_per_item : batch_default_rows(nb);
+}
+
 // Builds and launches one slot's batch from the admitted requests.
 int launch_slot(fgf_server *s, Slot &sl) {
     const int f = sl.reqs.front().r.field;
     const int nb = field_bytes(f);
     const size_t edw = entry_dwords(nb);
+    int dwords = 0, rows_per_item = 0;
+    tile_for(s, nb, dwords, rows_per_item);
     // Stripe geometry first (the work items need only lengths and counts),
     // then the block is sized, then the device pointers are filled in.
     std::vector<StripeDesc> stripes(sl.reqs.size());
@@ -332,15 +342,14 @@ int launch_slot(fgf_server *s, Slot &sl) {
     // tile uses items of a single tile, spreading a lone request over as many
     // waves as it has tiles (its latency is one wave's serial work); larger
     // launches use the configured multi-tile items.
-    const uint64_t tile_bytes = 256u * (uint64_t)s->cfg.dwords;
+    const uint64_t tile_bytes = 256u * (uint64_t)dwords;
     uint64_t tiles = 0;
     for (const StripeDesc &d : stripes)
         tiles += (d.row_len + tile_bytes - 1) / tile_bytes *
-                 ((d.m + (uint32_t)s->cfg.rows_per_item - 1) / (uint32_t)s->cfg.rows_per_item);
+                 ((d.m + (uint32_t)rows_per_item - 1) / (uint32_t)rows_per_item);
     const int subtiles =
         tiles <= (uint64_t)s->cus * (uint64_t)s->cfg.waves_per_cu ? 1 : s->cfg.subtiles;
-    std::vector<WorkItem> items =
-        build_work_items(stripes, s->cfg.dwords, s->cfg.rows_per_item, subtiles, 1);
+    std::vector<WorkItem> items = build_work_items(stripes, dwords, rows_per_item, subtiles, 1);
     const size_t off_str = align256(nptrs * sizeof(void *));
     const size_t off_items = off_str + align256(stripes.size() * sizeof(StripeDesc));
     const size_t total = blob_bytes(nptrs, stripes.size(), items.size());
@@ -373,7 +382,7 @@ int launch_slot(fgf_server *s, Slot &sl) {
     const uint32_t blocks = (uint32_t)((waves + wpb - 1) / wpb);
     BatchArgs a{(const StripeDesc *)(sl.d_blob + off_str), (const WorkItem *)(sl.d_blob + off_items),
                 (uint32_t)items.size(), blocks * wpb, (uint32_t)subtiles, sl.d_counter};
-    if ((e = launch_batch(f, s->cfg.dwords, s->cfg.rows_per_item, a, blocks, sl.stream)) ||
+    if ((e = launch_batch(f, dwords, rows_per_item, a, blocks, sl.stream)) ||
         (e = hipEventRecord(sl.done, sl.stream)))
         return sdev("batch launch", e);
     sl.busy = true;
@@ -523,8 +532,7 @@ namespace {
 int make_req(fgf_server *s, const fgf_request *r, Req &q) {
     if (!s || !r) return sfail("fgf_server_submit: null argument");
     const int f = r->field;
-    if (f != FGF_GF8B && f != FGF_GF8D && f != FGF_GF16 && f != FGF_FP8 && f != FGF_FP16)
-        return sfail("fgf_server_submit: field %d is not served (byte fields and 16-bit towers)", f);
+    if (!fgf::valid_field(f)) return sfail("fgf_server_submit: unknown field %d", f);
     if (r->k < 1 || r->m < 1 || r->k + r->m > 256)
         return sfail("fgf_server_submit: k = %u, m = %u outside 1 <= k, 1 <= m, k + m <= 256", r->k, r->m);
     if (r->k > 64) return sfail("fgf_server_submit: k = %u exceeds 64 sources", r->k);

@@ -262,10 +262,12 @@ typedef struct fgf_server_config {
     uint32_t slots;                /* concurrent launches in flight (streams), 1..16 */
     uint32_t max_batch_stripes;    /* requests per launch, >= 1 */
     uint32_t plan_cache_entries;   /* device plans kept, >= 1 */
-    int dwords, rows_per_item, subtiles, waves_per_cu; /* kernel shape, as fgf_batch_configure */
+    int dwords, rows_per_item, subtiles, waves_per_cu; /* kernel shape, as fgf_batch_configure;
+                                    4- and 8-byte fields take at least 8 dwords and their
+                                    own rows per item (4 and 2) */
 } fgf_server_config;
 typedef struct fgf_request {
-    fgf_field field;
+    fgf_field field;               /* any field; one launch serves one field */
     int reconstruct;               /* 0: write the m parity shards; 1: rebuild `erased` */
     uint32_t k, m;
     uint8_t *const *shards;        /* k + m device pointers: data shards then parity shards */

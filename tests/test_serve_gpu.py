@@ -54,6 +54,23 @@ def test_results_equal_alone_and_in_the_mix():
     srv.close()
 
 
+def test_every_field_width_is_served():
+    # 4- and 8-byte towers run with their own tile shape (8 dwords, 4 or 2
+    # rows per item) under the same server configuration as the byte fields.
+    from fgf_amd import FanPaar32, FanPaar64, Gf8B, Gf32, Gf64
+    mix = [(Gf32, 0.25), (Gf64, 0.25), (FanPaar32, 0.2), (FanPaar64, 0.2), (Gf8B, 0.1)]
+    specs = workloads.draw_serving_mix(mix, 64, seed=13, n_matrices=20, lmin=4096, lmax=64 << 10)
+    pool, expected = workloads.materialize_serving(specs, seed=14)
+    srv = Server(slots=3, max_batch_stripes=16, plan_cache_entries=32)
+    arrivals = np.sort(np.random.default_rng(2).uniform(0, 5e-4, len(specs))).tolist()
+    srv.replay([workloads.serve_request(pool, sp) for sp in specs], arrivals)
+    for sp, want in zip(specs, expected):
+        assert torch.equal(outputs(pool, sp), want), (sp.field.NAME, sp.k, sp.m, sp.erased)
+    st = srv.stats()
+    assert st["failed"] == 0 and st["completed"] == len(specs)
+    srv.close()
+
+
 def test_budget_admission_serialises_but_stays_exact():
     specs = workloads.draw_serving_mix([(Gf8D, 1.0)], 40, seed=9, n_matrices=6, lmin=8192,
                                        lmax=64 << 10)
