@@ -241,6 +241,27 @@ def _median_launch_ms(fn, reps: int = 20, warm: int = 3) -> float:
     return per[len(per) // 2]
 
 
+def _back_to_back_ms(fn, n: int = 50, reps: int = 5) -> float:
+    """Per-launch time of n launches issued back to back between two HIP
+    events (median over reps): the steady-state rate of a caller issuing
+    calls in a loop, without an event packet between launches."""
+    import torch
+    stream = torch.cuda.current_stream()
+    for _ in range(3):
+        fn()
+    torch.cuda.synchronize()
+    per = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(n):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        per.append(a.elapsed_time(b) / n)
+    return sorted(per)[len(per) // 2]
+
+
 def _leg(name: str, ms: float, nbytes: int, **extra) -> dict:
     gbps = nbytes / (ms * 1e-3) / 1e9
     return {"workload": name, "median_ms": round(ms, 4), "algorithmic_bytes": nbytes,
@@ -293,7 +314,9 @@ def bench_configs(seed: int = 31) -> dict:
     native = lambda: _lib.lib().fgf_dot_product_matrix_with(
         Gf8D.id, rows.data_ptr(), 4 * L, L, 4, plan._h, ptrs, lens, 10, sid)
     legs.append(_leg("Gf8D RS(10+4) encode, 1 MiB shards (cache-resident)", _median_launch_ms(run),
-                     14 * L, c_abi_median_ms=round(_median_launch_ms(native), 4)))
+                     14 * L, c_abi_median_ms=round(_median_launch_ms(native), 4),
+                     c_abi_back_to_back_ms=round(_back_to_back_ms(native), 4),
+                     python_back_to_back_ms=round(_back_to_back_ms(run), 4)))
     del src, rows
 
     # k = 64, m = 16, 4 MiB: scattered reconstruct of 16 erased slots.
