@@ -399,62 +399,25 @@ def pcie_inclusive(L: int, reps: int = 3) -> dict:
     want = h_rows.clone()
     del d_src, d_rows
     torch.cuda.empty_cache()
-    tp = pcie_pipelined(h_src, h_rows, plan, L, reps)
-    assert torch.equal(h_rows, want), "pipelined host-buffer encode differs from the one-shot encode"
+    # The library's host-buffer form: column chunks pipelined over three
+    # streams (H2D, encode, D2H at once).
+    srcs = list(h_src)
+    times = []
+    for _ in range(reps + 1):
+        h_rows.fill_(0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ops.dot_product_matrix_with_host(Gf8D, h_rows.view(-1), L, M, plan, srcs)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    assert torch.equal(h_rows, want), "host-buffer encode differs from the device-buffer encode"
+    tp = sorted(times[1:])[len(times[1:]) // 2]
     return {"GBps": round((K + M) * L / tp / 1e9, 1), "ms": round(tp * 1e3, 3),
             "sequential_GBps": round((K + M) * L / t / 1e9, 1), "sequential_ms": round(t * 1e3, 3),
             "note": "pinned host sources -> HBM, encode, parity -> pinned host; wall clock. "
-                    "GBps: column chunks pipelined over three streams (H2D, encode, D2H "
-                    "overlap; both copy directions busy at once); sequential: one H2D, "
-                    "one launch, one D2H"}
-
-
-def pcie_pipelined(h_src, h_rows, plan, L: int, reps: int, nchunks: int = 16,
-                   nslots: int = 3) -> float:
-    """Host-buffer encode as a pipeline over column chunks: chunk i's sources
-    go up while chunk i-1 is encoded and chunk i-2's parity comes down, each
-    stage on its own stream (default priority), through `nslots` rotating
-    device chunk buffers. Returns the median wall time of `reps` runs."""
-    import torch
-
-    from fgf_amd import Gf8D, ops
-    dev = torch.device("cuda:0")
-    C = (L + nchunks - 1) // nchunks
-    C = (C + 4095) // 4096 * 4096
-    bounds = [(c0, min(C, L - c0)) for c0 in range(0, L, C)]
-    d_src = [torch.empty((K, C), dtype=torch.uint8, device=dev) for _ in range(nslots)]
-    d_rows = [torch.empty((M, C), dtype=torch.uint8, device=dev) for _ in range(nslots)]
-    s_up, s_enc, s_down = torch.cuda.Stream(), torch.cuda.Stream(), torch.cuda.Stream()
-    up = [torch.cuda.Event() for _ in range(nslots)]
-    enc = [torch.cuda.Event() for _ in range(nslots)]
-    down = [torch.cuda.Event() for _ in range(nslots)]
-    times = []
-    for _ in range(reps + 1):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for i, (c0, c) in enumerate(bounds):
-            b = i % nslots
-            with torch.cuda.stream(s_up):
-                if i >= nslots:
-                    s_up.wait_event(down[b])          # slot b's parity is down
-                for t in range(K):
-                    d_src[b][t, :c].copy_(h_src[t, c0:c0 + c], non_blocking=True)
-                up[b].record(s_up)
-            s_enc.wait_event(up[b])
-            rows = d_rows[b].view(-1)[:M * c]                 # m x c, packed
-            ops.dot_product_matrix_with(Gf8D, rows, c, M, plan,
-                                        [d_src[b][t, :c] for t in range(K)], stream=s_enc.cuda_stream)
-            enc[b].record(s_enc)
-            with torch.cuda.stream(s_down):
-                s_down.wait_event(enc[b])
-                for j in range(M):
-                    h_rows[j, c0:c0 + c].copy_(rows[j * c:(j + 1) * c], non_blocking=True)
-                down[b].record(s_down)
-        torch.cuda.synchronize()
-        times.append(time.perf_counter() - t0)
-    del d_src, d_rows
-    torch.cuda.empty_cache()
-    return sorted(times[1:])[len(times[1:]) // 2]
+                    "GBps: fgf_dot_product_matrix_with_host (column chunks pipelined over "
+                    "three streams, both copy directions busy at once); sequential: one "
+                    "H2D, one device-buffer launch, one D2H"}
 
 
 def bench_serving(n_requests: int = 4096, seed: int = 77,

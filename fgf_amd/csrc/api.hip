@@ -4,6 +4,7 @@
 // reference's message in fgf_last_error().
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -738,6 +739,141 @@ int fgf_dot_product_matrix_with(fgf_field f, uint8_t *rows, size_t rows_len, siz
     auto r = contiguous_rows(rows, nrows, row_len);
     return run_matrix(f, plan->d_tab, nrows * e, e, r.data(), nrows, srcs, nsrcs, row_len, true,
                       S(stream));
+}
+
+namespace {
+
+// Host-buffer pipeline, one per device: column chunks of the sources go up
+// on one stream while the previous chunk is encoded on a second and the one
+// before comes back down on a third, through HOST_SLOTS rotating device
+// chunk buffers. Streams are created once (default priority, non-blocking)
+// and ordered after the caller's stream with events.
+constexpr int HOST_SLOTS = 3;
+constexpr int HOST_MAX_DEVICES = 64;
+constexpr size_t HOST_CHUNK = 8u << 20;       // bytes per shard per chunk
+constexpr size_t HOST_SLOT_CAP = 256u << 20;  // device bytes per slot
+
+struct HostPipe {
+    std::mutex mu;
+    bool ready = false;
+    hipStream_t st[3] = {};                   // up, encode, down
+    hipEvent_t ev[3][HOST_SLOTS] = {};        // [stage][slot] done
+    bool used[HOST_SLOTS] = {};
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    uint8_t *buf = nullptr;
+    size_t cap = 0;
+};
+
+HostPipe g_pipes[HOST_MAX_DEVICES];
+
+hipError_t pipe_init(HostPipe &p) {
+    if (p.ready) return hipSuccess;
+    hipError_t e;
+    for (hipStream_t &st : p.st)
+        if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess) return e;
+    for (auto &stage : p.ev)
+        for (hipEvent_t &x : stage)
+            if ((e = hipEventCreateWithFlags(&x, hipEventDisableTiming)) != hipSuccess) return e;
+    if ((e = hipEventCreateWithFlags(&p.ev_in, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&p.ev_out, hipEventDisableTiming)) != hipSuccess)
+        return e;
+    p.ready = true;
+    return hipSuccess;
+}
+
+// Grows the slot buffers; the old buffer is freed only after every stage
+// that could still read or write it has finished.
+hipError_t pipe_reserve(HostPipe &p, size_t bytes) {
+    if (bytes <= p.cap) return hipSuccess;
+    hipError_t e;
+    for (hipStream_t st : p.st)
+        if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+    if (p.buf && (e = hipFree(p.buf)) != hipSuccess) return e;
+    p.buf = nullptr;
+    p.cap = 0;
+    if ((e = hipMalloc((void **)&p.buf, bytes)) != hipSuccess) return e;
+    p.cap = bytes;
+    return hipSuccess;
+}
+
+}  // namespace
+
+int fgf_dot_product_matrix_with_host(fgf_field f, uint8_t *rows, size_t rows_len, size_t row_len,
+                                     size_t nrows, const fgf_plan *plan, const uint8_t *const *srcs,
+                                     const size_t *src_lens, size_t nsrcs, void *stream) {
+    FIELD_GUARD(f);
+    int rc = 0;
+    size_t used;
+    if (!check_matrix_plan(f, "dot_product_matrix_with_host", rows_len, row_len, nrows, plan,
+                           src_lens, nsrcs, &used, &rc))
+        return rc;
+    if (nrows == 0) return FGF_OK;
+    const hipStream_t s = S(stream);
+    hipError_t e;
+    if (nsrcs == 0) {
+        // No terms: zero rows[..used] once earlier work on the stream is done.
+        if ((e = hipStreamSynchronize(s)) != hipSuccess) return device_fail("hipStreamSynchronize", e);
+        std::memset(rows, 0, used);
+        return FGF_OK;
+    }
+    if (row_len == 0) return FGF_OK;
+    int dev = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return device_fail("hipGetDevice", e);
+    if (dev < 0 || dev >= HOST_MAX_DEVICES) return fail("dot_product_matrix_with_host: device %d", dev);
+    HostPipe &p = g_pipes[dev];
+    std::lock_guard<std::mutex> lk(p.mu);
+    if ((e = pipe_init(p)) != hipSuccess) return device_fail("host pipeline streams", e);
+
+    const size_t k = nsrcs, m = nrows;
+    // Chunk width: whole elements, at most HOST_CHUNK per shard and
+    // HOST_SLOT_CAP per slot, a multiple of 4 KiB unless one chunk is all.
+    size_t C = std::min(HOST_CHUNK, std::max<size_t>(4096, HOST_SLOT_CAP / (k + m)));
+    C = C / 4096 * 4096;
+    if (C >= row_len) C = row_len;
+    const size_t slot = (k + m) * C;
+    if ((e = pipe_reserve(p, HOST_SLOTS * slot)) != hipSuccess) return device_fail("host pipeline buffers", e);
+
+    hipStream_t up = p.st[0], enc = p.st[1], down = p.st[2];
+    if ((e = hipEventRecord(p.ev_in, s)) != hipSuccess || (e = hipStreamWaitEvent(up, p.ev_in, 0)) != hipSuccess)
+        return device_fail("host pipeline ordering", e);
+    const size_t edw = entry_dw(plan->nb);
+    std::vector<const uint8_t *> dsrc(k);
+    std::vector<uint8_t *> drow(m);
+    size_t i = 0;
+    for (size_t c0 = 0; c0 < row_len; c0 += C, ++i) {
+        const size_t c = std::min(C, row_len - c0);
+        const int b = (int)(i % HOST_SLOTS);
+        uint8_t *base = p.buf + (size_t)b * slot;
+        for (size_t t = 0; t < k; ++t) dsrc[t] = base + t * C;
+        for (size_t j = 0; j < m; ++j) drow[j] = base + (k + j) * C;
+        // Up: wait until the slot's previous parity has come down.
+        if (p.used[b] && (e = hipStreamWaitEvent(up, p.ev[2][b], 0)) != hipSuccess)
+            return device_fail("host pipeline ordering", e);
+        for (size_t t = 0; t < k; ++t)
+            if ((e = hipMemcpyAsync((void *)dsrc[t], srcs[t] + c0, c, hipMemcpyHostToDevice, up)) != hipSuccess)
+                return device_fail("hipMemcpyAsync H2D", e);
+        if ((e = hipEventRecord(p.ev[0][b], up)) != hipSuccess ||
+            (e = hipStreamWaitEvent(enc, p.ev[0][b], 0)) != hipSuccess)
+            return device_fail("host pipeline ordering", e);
+        if ((rc = run_matrix(f, plan->d_tab, m * edw, edw, drow.data(), m, dsrc.data(), k, c, true, enc)))
+            return rc;
+        if ((e = hipEventRecord(p.ev[1][b], enc)) != hipSuccess ||
+            (e = hipStreamWaitEvent(down, p.ev[1][b], 0)) != hipSuccess)
+            return device_fail("host pipeline ordering", e);
+        for (size_t j = 0; j < m; ++j)
+            if ((e = hipMemcpyAsync(rows + j * row_len + c0, drow[j], c, hipMemcpyDeviceToHost, down)) != hipSuccess)
+                return device_fail("hipMemcpyAsync D2H", e);
+        if ((e = hipEventRecord(p.ev[2][b], down)) != hipSuccess) return device_fail("host pipeline ordering", e);
+        p.used[b] = true;
+    }
+    if ((e = hipEventRecord(p.ev_out, down)) != hipSuccess || (e = hipStreamWaitEvent(s, p.ev_out, 0)) != hipSuccess)
+        return device_fail("host pipeline ordering", e);
+    return FGF_OK;
+}
+
+int fgf_stream_synchronize(void *stream) {
+    const hipError_t e = hipStreamSynchronize(S(stream));
+    return e == hipSuccess ? FGF_OK : device_fail("hipStreamSynchronize", e);
 }
 
 namespace {

@@ -23,7 +23,8 @@ __all__ = [
     "mul_into", "mul_into_with", "mul_assign", "mul_assign_with", "mul_add_scatter",
     "mul_add_scatter_with", "mul_add_gather", "mul_add_gather_with", "dot_product",
     "dot_product_with", "mul_add_matrix", "mul_add_matrix_with", "dot_product_matrix",
-    "dot_product_matrix_with", "mul_add_matrix_scattered", "mul_add_matrix_scattered_with",
+    "dot_product_matrix_with", "dot_product_matrix_with_host", "mul_add_matrix_scattered",
+    "mul_add_matrix_scattered_with",
     "mul_elementwise", "pack", "unpack", "pack_to_vec", "Panic",
 ]
 
@@ -322,6 +323,37 @@ def dot_product_matrix_with(field: Field, rows, row_len: int, nrows: int, plan: 
     ptrs, lens = _ptr_array(srcs)
     _check(_lib.lib().fgf_dot_product_matrix_with(field.id, r, rl, row_len, nrows, plan._h, ptrs,
                                                   lens, len(srcs), _stream(stream, rows, *srcs)))
+
+
+def _hbuf(x) -> tuple[int, int]:
+    """A host byte range: a numpy array, a CPU tensor (pinned or not) or a Buf."""
+    if isinstance(x, Buf):
+        return x.ptr, x.nbytes
+    if hasattr(x, "__array_interface__") and not hasattr(x, "data_ptr"):
+        if not x.flags["C_CONTIGUOUS"]:
+            raise Panic("buffer must be a contiguous byte range")
+        return x.ctypes.data, x.nbytes
+    if getattr(x, "is_cuda", False):
+        raise Panic("host-buffer form takes host memory, got a device tensor")
+    return _buf(x)
+
+
+def dot_product_matrix_with_host(field: Field, rows, row_len: int, nrows: int, plan: Plan,
+                                 srcs: Sequence, stream=None) -> None:
+    """dot_product_matrix_with over host buffers (src/ops.rs:823-867): the
+    library pipelines column chunks up, through the kernel and back down.
+    Completes on `stream` (default: the current stream of the current
+    device); synchronize it before reading `rows`."""
+    r, rl = _hbuf(rows)
+    pl = [_hbuf(x) for x in srcs]
+    n = max(len(pl), 1)
+    ptrs = (ctypes.c_void_p * n)(*[q for q, _ in pl])
+    lens = (ctypes.c_size_t * n)(*[ln for _, ln in pl])
+    if stream is None:
+        import torch
+        stream = _current_stream(torch.cuda.current_device())
+    _check(_lib.lib().fgf_dot_product_matrix_with_host(field.id, r, rl, row_len, nrows, plan._h,
+                                                       ptrs, lens, len(srcs), int(stream)))
 
 
 def mul_add_matrix_scattered(field: Field, dst, row_len: int, row_starts: Sequence[int], terms,

@@ -173,6 +173,24 @@ int fgf_dot_product_matrix_with(fgf_field field, uint8_t *rows, size_t rows_len,
                                 size_t row_len, size_t nrows, const fgf_plan *plan,
                                 const uint8_t *const *srcs, const size_t *src_lens,
                                 size_t nsrcs, void *stream);
+/* fgf_dot_product_matrix_with over HOST buffers (src/ops.rs:823-867, the
+ * reference's own signature takes host slices): `rows` and every srcs[t] are
+ * host pointers. Column chunks go up, are encoded and come back down on
+ * three library streams at once (pinned host memory gives the full PCIe
+ * rate; pageable memory works through the runtime's staging). Ordered after
+ * earlier work on `stream`, and `stream` waits for it: synchronize `stream`
+ * before reading rows or reusing the sources. Same checks and error text as
+ * the device form; no terms zero rows[..nrows*row_len] after synchronizing
+ * `stream`. Concurrent calls on one device are serialised. */
+int fgf_dot_product_matrix_with_host(fgf_field field, uint8_t *rows, size_t rows_len,
+                                     size_t row_len, size_t nrows, const fgf_plan *plan,
+                                     const uint8_t *const *srcs, const size_t *src_lens,
+                                     size_t nsrcs, void *stream);
+/* Waits for the work queued on `stream` (NULL: the null stream). No
+ * counterpart in the reference, whose ops return with their output written:
+ * a caller keeping that contract over the host-buffer form calls this after
+ * it, without binding HIP itself. */
+int fgf_stream_synchronize(void *stream);
 /* mul_add_matrix_scattered (src/ops.rs:887-941): row j at dst + row_starts[j]. */
 int fgf_mul_add_matrix_scattered(fgf_field field, uint8_t *dst, size_t dst_len, size_t row_len,
                                  const size_t *row_starts, size_t nrows,

@@ -232,6 +232,58 @@ def test_matrix_every_source_count_mod_4_over_ragged_tiles(f):
             assert np.array_equal(host(acc), want), (row_len, k, "accumulate")
 
 
+@pytest.mark.parametrize("f", [Gf8D, Gf32, FanPaar64], ids=lambda f: NAMES[f.id])
+def test_host_buffer_form_matches_device_form(f):
+    # The host form pipelines 8 MiB column chunks through three streams and
+    # three device slots: a length of two full chunks plus a ragged tail
+    # reuses slots, and a one-chunk length does not. Pinned (CPU tensor) and
+    # pageable (numpy) host memory both work.
+    name = NAMES[f.id]
+    k, m = 10, 4
+    for row_len, pinned in (((17 << 20) + 4096 + 64, True), (4096 + 48, False)):
+        srcs = [noise(row_len, 0xD00 + t) for t in range(k)]
+        coeffs = rand_coeffs(f, k * m, 0xE00 + f.id)
+        plan = ops.Plan.matrix(f, k, m, coeffs)
+        want = torch.empty(m * row_len, dtype=torch.uint8, device="cuda")
+        ops.dot_product_matrix_with(f, want, row_len, m, plan, [dev(x) for x in srcs])
+        want = host(want)
+        if pinned:
+            h_srcs = [torch.from_numpy(x).pin_memory() for x in srcs]
+            h_rows = torch.full((m * row_len,), 0xA5, dtype=torch.uint8).pin_memory()
+        else:
+            h_srcs = srcs
+            h_rows = np.full(m * row_len, 0xA5, np.uint8)
+        ops.dot_product_matrix_with_host(f, h_rows, row_len, m, plan, h_srcs)
+        torch.cuda.synchronize()
+        got = h_rows.numpy() if pinned else h_rows
+        assert np.array_equal(got, want), (row_len, pinned)
+        for off in (0, row_len - 256):              # the device form itself, against the oracle
+            exp = oracle_matrix(name, 256, m, coeffs, [x[off:off + 256] for x in srcs])
+            assert np.array_equal(want.reshape(m, row_len)[:, off:off + 256].reshape(-1), exp)
+
+
+def test_host_buffer_form_back_to_back_calls_and_empty_terms():
+    # Two calls queued without a synchronize reuse the same device slots; the
+    # second must not overwrite a slot the first has not finished with.
+    k, m = 6, 3
+    a_len, b_len = (9 << 20) + 4096, (5 << 20) + 512
+    plan = ops.Plan.matrix(Gf8D, k, m, rand_coeffs(Gf8D, k * m, 0xF01))
+    out = []
+    for n, seed in ((a_len, 0xF10), (b_len, 0xF20)):
+        srcs = [torch.from_numpy(noise(n, seed + t)).pin_memory() for t in range(k)]
+        rows = torch.zeros(m * n, dtype=torch.uint8).pin_memory()
+        ops.dot_product_matrix_with_host(Gf8D, rows, n, m, plan, srcs)
+        out.append((n, srcs, rows))
+    torch.cuda.synchronize()
+    for n, srcs, rows in out:
+        want = torch.empty(m * n, dtype=torch.uint8, device="cuda")
+        ops.dot_product_matrix_with(Gf8D, want, n, m, plan, [x.cuda() for x in srcs])
+        assert torch.equal(rows, want.cpu()), n
+    zeros = np.full(64 * 2 + 8, 0x5A, np.uint8)
+    ops.dot_product_matrix_with_host(Gf8D, zeros, 64, 2, ops.Plan.matrix(Gf8D, 0, 2, []), [])
+    assert not zeros[:128].any() and (zeros[128:] == 0x5A).all()
+
+
 @pytest.mark.parametrize("f", ALL_FIELDS, ids=lambda f: NAMES[f.id])
 def test_matrix_scattered_matches_contiguous(f):
     # tests/ops.rs:406-478: rows separated by element-aligned gaps of 7
