@@ -798,19 +798,22 @@ hipError_t pipe_reserve(HostPipe &p, size_t bytes) {
 
 }  // namespace
 
-int fgf_dot_product_matrix_with_host(fgf_field f, uint8_t *rows, size_t rows_len, size_t row_len,
-                                     size_t nrows, const fgf_plan *plan, const uint8_t *const *srcs,
-                                     const size_t *src_lens, size_t nsrcs, void *stream) {
-    FIELD_GUARD(f);
+namespace {
+
+// The host-buffer pipeline over column chunks. Accumulating (overwrite
+// false), each chunk of the destination rows goes up with its sources and
+// comes back down with the products folded in.
+int host_matrix(int f, const char *name, uint8_t *rows, size_t rows_len, size_t row_len,
+                size_t nrows, const fgf_plan *plan, const uint8_t *const *srcs,
+                const size_t *src_lens, size_t nsrcs, bool overwrite, hipStream_t s) {
     int rc = 0;
     size_t used;
-    if (!check_matrix_plan(f, "dot_product_matrix_with_host", rows_len, row_len, nrows, plan,
-                           src_lens, nsrcs, &used, &rc))
+    if (!check_matrix_plan(f, name, rows_len, row_len, nrows, plan, src_lens, nsrcs, &used, &rc))
         return rc;
     if (nrows == 0) return FGF_OK;
-    const hipStream_t s = S(stream);
     hipError_t e;
     if (nsrcs == 0) {
+        if (!overwrite) return FGF_OK;
         // No terms: zero rows[..used] once earlier work on the stream is done.
         if ((e = hipStreamSynchronize(s)) != hipSuccess) return device_fail("hipStreamSynchronize", e);
         std::memset(rows, 0, used);
@@ -819,14 +822,14 @@ int fgf_dot_product_matrix_with_host(fgf_field f, uint8_t *rows, size_t rows_len
     if (row_len == 0) return FGF_OK;
     int dev = 0;
     if ((e = hipGetDevice(&dev)) != hipSuccess) return device_fail("hipGetDevice", e);
-    if (dev < 0 || dev >= HOST_MAX_DEVICES) return fail("dot_product_matrix_with_host: device %d", dev);
+    if (dev < 0 || dev >= HOST_MAX_DEVICES) return fail("%s: device %d", name, dev);
     HostPipe &p = g_pipes[dev];
     std::lock_guard<std::mutex> lk(p.mu);
     if ((e = pipe_init(p)) != hipSuccess) return device_fail("host pipeline streams", e);
 
     const size_t k = nsrcs, m = nrows;
-    // Chunk width: whole elements, at most HOST_CHUNK per shard and
-    // HOST_SLOT_CAP per slot, a multiple of 4 KiB unless one chunk is all.
+    // Chunk width: at most HOST_CHUNK per shard and HOST_SLOT_CAP per slot,
+    // a multiple of 4 KiB (whole elements) unless one chunk is all.
     size_t C = std::min(HOST_CHUNK, std::max<size_t>(4096, HOST_SLOT_CAP / (k + m)));
     C = C / 4096 * 4096;
     if (C >= row_len) C = row_len;
@@ -846,16 +849,21 @@ int fgf_dot_product_matrix_with_host(fgf_field f, uint8_t *rows, size_t rows_len
         uint8_t *base = p.buf + (size_t)b * slot;
         for (size_t t = 0; t < k; ++t) dsrc[t] = base + t * C;
         for (size_t j = 0; j < m; ++j) drow[j] = base + (k + j) * C;
-        // Up: wait until the slot's previous parity has come down.
+        // Up: wait until the slot's previous rows have come down.
         if (p.used[b] && (e = hipStreamWaitEvent(up, p.ev[2][b], 0)) != hipSuccess)
             return device_fail("host pipeline ordering", e);
         for (size_t t = 0; t < k; ++t)
             if ((e = hipMemcpyAsync((void *)dsrc[t], srcs[t] + c0, c, hipMemcpyHostToDevice, up)) != hipSuccess)
                 return device_fail("hipMemcpyAsync H2D", e);
+        if (!overwrite)
+            for (size_t j = 0; j < m; ++j)
+                if ((e = hipMemcpyAsync(drow[j], rows + j * row_len + c0, c, hipMemcpyHostToDevice, up)) !=
+                    hipSuccess)
+                    return device_fail("hipMemcpyAsync H2D", e);
         if ((e = hipEventRecord(p.ev[0][b], up)) != hipSuccess ||
             (e = hipStreamWaitEvent(enc, p.ev[0][b], 0)) != hipSuccess)
             return device_fail("host pipeline ordering", e);
-        if ((rc = run_matrix(f, plan->d_tab, m * edw, edw, drow.data(), m, dsrc.data(), k, c, true, enc)))
+        if ((rc = run_matrix(f, plan->d_tab, m * edw, edw, drow.data(), m, dsrc.data(), k, c, overwrite, enc)))
             return rc;
         if ((e = hipEventRecord(p.ev[1][b], enc)) != hipSuccess ||
             (e = hipStreamWaitEvent(down, p.ev[1][b], 0)) != hipSuccess)
@@ -869,6 +877,24 @@ int fgf_dot_product_matrix_with_host(fgf_field f, uint8_t *rows, size_t rows_len
     if ((e = hipEventRecord(p.ev_out, down)) != hipSuccess || (e = hipStreamWaitEvent(s, p.ev_out, 0)) != hipSuccess)
         return device_fail("host pipeline ordering", e);
     return FGF_OK;
+}
+
+}  // namespace
+
+int fgf_dot_product_matrix_with_host(fgf_field f, uint8_t *rows, size_t rows_len, size_t row_len,
+                                     size_t nrows, const fgf_plan *plan, const uint8_t *const *srcs,
+                                     const size_t *src_lens, size_t nsrcs, void *stream) {
+    FIELD_GUARD(f);
+    return host_matrix(f, "dot_product_matrix_with_host", rows, rows_len, row_len, nrows, plan, srcs,
+                       src_lens, nsrcs, true, S(stream));
+}
+
+int fgf_mul_add_matrix_with_host(fgf_field f, uint8_t *rows, size_t rows_len, size_t row_len,
+                                 size_t nrows, const fgf_plan *plan, const uint8_t *const *srcs,
+                                 const size_t *src_lens, size_t nsrcs, void *stream) {
+    FIELD_GUARD(f);
+    return host_matrix(f, "mul_add_matrix_with_host", rows, rows_len, row_len, nrows, plan, srcs,
+                       src_lens, nsrcs, false, S(stream));
 }
 
 int fgf_stream_synchronize(void *stream) {

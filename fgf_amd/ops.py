@@ -23,7 +23,8 @@ __all__ = [
     "mul_into", "mul_into_with", "mul_assign", "mul_assign_with", "mul_add_scatter",
     "mul_add_scatter_with", "mul_add_gather", "mul_add_gather_with", "dot_product",
     "dot_product_with", "mul_add_matrix", "mul_add_matrix_with", "dot_product_matrix",
-    "dot_product_matrix_with", "dot_product_matrix_with_host", "mul_add_matrix_scattered",
+    "dot_product_matrix_with", "dot_product_matrix_with_host", "mul_add_matrix_with_host",
+    "mul_add_matrix_scattered",
     "mul_add_matrix_scattered_with",
     "mul_elementwise", "pack", "unpack", "pack_to_vec", "Panic",
 ]
@@ -338,12 +339,8 @@ def _hbuf(x) -> tuple[int, int]:
     return _buf(x)
 
 
-def dot_product_matrix_with_host(field: Field, rows, row_len: int, nrows: int, plan: Plan,
-                                 srcs: Sequence, stream=None) -> None:
-    """dot_product_matrix_with over host buffers (src/ops.rs:823-867): the
-    library pipelines column chunks up, through the kernel and back down.
-    Completes on `stream` (default: the current stream of the current
-    device); synchronize it before reading `rows`."""
+def _host_matrix(fn, field: Field, rows, row_len: int, nrows: int, plan: Plan, srcs: Sequence,
+                 stream) -> None:
     r, rl = _hbuf(rows)
     pl = [_hbuf(x) for x in srcs]
     n = max(len(pl), 1)
@@ -352,8 +349,25 @@ def dot_product_matrix_with_host(field: Field, rows, row_len: int, nrows: int, p
     if stream is None:
         import torch
         stream = _current_stream(torch.cuda.current_device())
-    _check(_lib.lib().fgf_dot_product_matrix_with_host(field.id, r, rl, row_len, nrows, plan._h,
-                                                       ptrs, lens, len(srcs), int(stream)))
+    _check(fn(field.id, r, rl, row_len, nrows, plan._h, ptrs, lens, len(srcs), int(stream)))
+
+
+def dot_product_matrix_with_host(field: Field, rows, row_len: int, nrows: int, plan: Plan,
+                                 srcs: Sequence, stream=None) -> None:
+    """dot_product_matrix_with over host buffers (src/ops.rs:823-867): the
+    library pipelines column chunks up, through the kernel and back down.
+    Completes on `stream` (default: the current stream of the current
+    device); synchronize it before reading `rows`."""
+    _host_matrix(_lib.lib().fgf_dot_product_matrix_with_host, field, rows, row_len, nrows, plan,
+                 srcs, stream)
+
+
+def mul_add_matrix_with_host(field: Field, rows, row_len: int, nrows: int, plan: Plan,
+                             srcs: Sequence, stream=None) -> None:
+    """mul_add_matrix_with over host buffers (src/ops.rs:720-760): rows ^= the
+    products, chunks of the rows going up beside their sources."""
+    _host_matrix(_lib.lib().fgf_mul_add_matrix_with_host, field, rows, row_len, nrows, plan,
+                 srcs, stream)
 
 
 def mul_add_matrix_scattered(field: Field, dst, row_len: int, row_starts: Sequence[int], terms,

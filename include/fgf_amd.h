@@ -186,6 +186,13 @@ int fgf_dot_product_matrix_with_host(fgf_field field, uint8_t *rows, size_t rows
                                      size_t row_len, size_t nrows, const fgf_plan *plan,
                                      const uint8_t *const *srcs, const size_t *src_lens,
                                      size_t nsrcs, void *stream);
+/* fgf_mul_add_matrix_with over HOST buffers (src/ops.rs:720-760): rows ^=
+ * the products, the same pipeline with each chunk of the destination rows
+ * going up beside its sources. No terms: a no-op. */
+int fgf_mul_add_matrix_with_host(fgf_field field, uint8_t *rows, size_t rows_len,
+                                 size_t row_len, size_t nrows, const fgf_plan *plan,
+                                 const uint8_t *const *srcs, const size_t *src_lens,
+                                 size_t nsrcs, void *stream);
 /* Waits for the work queued on `stream` (NULL: the null stream). No
  * counterpart in the reference, whose ops return with their output written:
  * a caller keeping that contract over the host-buffer form calls this after

@@ -393,6 +393,17 @@ void dot_product_matrix_with_host(Bytes rows, std::size_t row_len, std::size_t n
                                                    srcs.size(), stream));
 }
 
+/* mul_add_matrix_with over HOST buffers, the same pipeline accumulating. */
+template <class F>
+void mul_add_matrix_with_host(Bytes rows, std::size_t row_len, std::size_t nrows,
+                              const Plan<F> &plan, const std::vector<CBytes> &srcs,
+                              void *stream = nullptr) {
+    detail::Srcs s(srcs);
+    detail::check(fgf_mul_add_matrix_with_host(F::ID, rows.ptr, rows.len, row_len, nrows,
+                                               plan.raw(), s.ptrs.data(), s.lens.data(),
+                                               srcs.size(), stream));
+}
+
 /* mul_add_matrix_scattered (src/ops.rs:887-941): row j at dst + row_starts[j]. */
 template <class F>
 void mul_add_matrix_scattered(Bytes dst, std::size_t row_len,

@@ -282,6 +282,31 @@ def test_host_buffer_form_back_to_back_calls_and_empty_terms():
     zeros = np.full(64 * 2 + 8, 0x5A, np.uint8)
     ops.dot_product_matrix_with_host(Gf8D, zeros, 64, 2, ops.Plan.matrix(Gf8D, 0, 2, []), [])
     assert not zeros[:128].any() and (zeros[128:] == 0x5A).all()
+    same = zeros.copy()
+    ops.mul_add_matrix_with_host(Gf8D, same, 64, 2, ops.Plan.matrix(Gf8D, 0, 2, []), [])
+    torch.cuda.synchronize()
+    assert np.array_equal(same, zeros)
+
+
+@pytest.mark.parametrize("f", [Gf8D, Gf16, Gf64], ids=lambda f: NAMES[f.id])
+def test_host_buffer_accumulate_matches_device_form(f):
+    k, m = 7, 5
+    for row_len, pinned in (((17 << 20) + 4096 + 64, True), (2048 + 16, False)):
+        srcs = [noise(row_len, 0x1100 + t) for t in range(k)]
+        init = noise(m * row_len, 0x1200 + f.id)
+        plan = ops.Plan.matrix(f, k, m, rand_coeffs(f, k * m, 0x1300 + f.id))
+        want = dev(init)
+        ops.mul_add_matrix_with(f, want, row_len, m, plan, [dev(x) for x in srcs])
+        want = host(want)
+        if pinned:
+            h_srcs = [torch.from_numpy(x).pin_memory() for x in srcs]
+            h_rows = torch.from_numpy(init.copy()).pin_memory()
+        else:
+            h_srcs, h_rows = srcs, init.copy()
+        ops.mul_add_matrix_with_host(f, h_rows, row_len, m, plan, h_srcs)
+        torch.cuda.synchronize()
+        got = h_rows.numpy() if pinned else h_rows
+        assert np.array_equal(got, want), (row_len, pinned)
 
 
 @pytest.mark.parametrize("f", ALL_FIELDS, ids=lambda f: NAMES[f.id])
