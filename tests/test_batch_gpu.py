@@ -125,6 +125,30 @@ def test_batch_mixed_accumulate_and_overwrite_across_tile_shapes(field):
             assert np.array_equal(rows.cpu().numpy(), exp), (cfg, k, m, L, ow)
 
 
+@pytest.mark.parametrize("field", [Gf8D, Gf16], ids=["gf8d", "gf16"])
+def test_batch_many_sources_and_ragged_row_groups(field):
+    # 100 sources stream through the ring with scalar table reads (no LDS
+    # staging, so no source chunking), and 11 rows split into a full row
+    # group and a ragged one, in both modes.
+    name = "gf8d" if field is Gf8D else "gf16"
+    g = np.random.default_rng(44)
+    k, m, L = 100, 11, 3 * 2048 + 16
+    srcs = [g.integers(0, 256, L, dtype=np.uint8) for _ in range(k)]
+    vals = [int(v) for v in g.integers(0, 1 << field.BITS, k * m, dtype=np.uint64)]
+    plan = ops.Plan.matrix(field, k, m, vals)
+    d_srcs = [torch.from_numpy(x).cuda() for x in srcs]
+    for ow in (True, False):
+        init = g.integers(0, 256, m * L, dtype=np.uint8)
+        rows = torch.from_numpy(init.copy()).cuda()
+        b = Batch(field, [Stripe(plan, d_srcs, [rows[j * L:(j + 1) * L] for j in range(m)], L, ow)])
+        b.configure(8, 8, 2, 8, 1)
+        b.run()
+        torch.cuda.synchronize()
+        exp = init.copy()
+        ol.mul_add_matrix(name, exp, L, m, vals, srcs, ow)
+        assert np.array_equal(rows.cpu().numpy(), exp), ow
+
+
 def test_batch_rejects_misaligned_and_overlapping_rows():
     plan = ops.Plan.matrix(Gf8D, 1, 2, [3, 5])
     buf = torch.zeros(8192, dtype=torch.uint8, device="cuda")
