@@ -134,8 +134,11 @@ def bench_batched(n_stripes: int = 4096, seed: int = 4096, reps: int = 5) -> dic
     total = sum(sp.algorithmic_bytes for sp in specs)
     stream = torch.cuda.current_stream()
 
-    def timed(fn):
-        fn()
+    def timed(fn, warm=3):
+        # A few untimed launches first: after the host-bound per-call leg the
+        # GPU's clocks take a few milliseconds of load to come back up.
+        for _ in range(warm):
+            fn()
         torch.cuda.synchronize()
         times = []
         for _ in range(reps):
