@@ -94,13 +94,22 @@ def tune_batch(batch, reps: int = 3, use_cache: bool = True) -> dict:
             batch.configure(*cfg)
             trials[cfg] = _time(batch, reps)
     d, r = min(trials, key=trials.get)[:2]
-    # Stage 2: item size and residency for the winning tile.
-    for sub in (1, 4, 16):
-        for w in (8, 16, 32):
+    # Stage 2: item size and residency for the winning tile. One-tile items
+    # pay an item's fixed cost (claim, descriptor reads, the first load's
+    # full latency) per tile and large items leave a ragged tail, so the
+    # middle sizes are the ones worth timing.
+    for sub in (2, 4, 8):
+        for w in (8, 12, 16):
             cfg = (d, r, sub, w, 1)
             if cfg not in trials:
                 batch.configure(*cfg)
                 trials[cfg] = _time(batch, reps)
+    # Stage 3: the work order for the winner (stripe order keeps a stripe's
+    # chunks adjacent in the queue; largest-first balances the tail).
+    d, r, sub, w, _ = min(trials, key=trials.get)
+    cfg = (d, r, sub, w, 0)
+    batch.configure(*cfg)
+    trials[cfg] = _time(batch, reps)
     best = min(trials, key=trials.get)
     batch.configure(*best)
     if use_cache:

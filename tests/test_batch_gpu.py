@@ -149,6 +149,28 @@ def test_batch_many_sources_and_ragged_row_groups(field):
         assert np.array_equal(rows.cpu().numpy(), exp), ow
 
 
+def test_autotuner_picks_a_valid_shape_and_keeps_outputs_exact(tmp_path, monkeypatch):
+    from fgf_amd import autotune
+    monkeypatch.setenv(autotune.CACHE_ENV, str(tmp_path / "tune.json"))
+    specs = workloads.draw_uneven_mix(Gf8D, 48, seed=31, lmin=4096, lmax=64 << 10)
+    pool, saved = workloads.materialize(Gf8D, specs, seed=32)
+    batch, plans = build(Gf8D, specs, pool)
+    want = [out.clone() for _, out in per_call(Gf8D, specs, pool, plans)]
+    res = autotune.tune_batch(batch, reps=1)
+    assert not res["cached"] and res["trials"] >= 10
+    d, r, sub, w, order = res["config"]
+    assert d in (4, 8) and r in (4, 8) and 1 <= sub <= 64 and 1 <= w <= 32 and order in (0, 1)
+    for sp in specs:
+        for x in workloads.stripe_buffers(pool, sp)[1]:
+            x.fill_(0xA5)
+    batch.run()
+    torch.cuda.synchronize()
+    for i, sp in enumerate(specs):
+        assert torch.equal(torch.cat(workloads.stripe_buffers(pool, sp)[1]), want[i]), i
+    again = autotune.tune_batch(batch)                  # now served from the cache file
+    assert again["cached"] and again["config"] == res["config"]
+
+
 def test_batch_rejects_misaligned_and_overlapping_rows():
     plan = ops.Plan.matrix(Gf8D, 1, 2, [3, 5])
     buf = torch.zeros(8192, dtype=torch.uint8, device="cuda")
