@@ -53,13 +53,14 @@ def measured_traffic(shard_bytes: int):
     return None
 
 
-def cpu_baseline(seconds_budget: float = 12.0) -> dict:
+def cpu_baseline(seconds_budget: float = 16.0) -> dict:
     """Time the C port of the reference's x86 Gf8D matrix path on host cores.
 
     Sample: RS(10+4) with 16 MiB shards (the headline shape at 1/6.4 of the
-    row length), timed like benches/kernels.rs:41-64 (warm-up, then the
-    median of repeated runs), first on 1 core (the reference is
-    single-threaded per call), then on all cores.
+    row length), timed like benches/kernels.rs:41-64: 16 warm-up calls, then
+    samples of 32 calls each, at most 64 samples or half the time budget per
+    core count, and the median sample. First on 1 core (the reference is
+    single-threaded per call), then on all cores. About 12 s of CPU work.
     """
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import numpy as np
@@ -76,14 +77,17 @@ def cpu_baseline(seconds_budget: float = 12.0) -> dict:
     name = "v3_gfni_crypto" if backend == 0 else "v3"
     results = {}
     ncores = len(os.sched_getaffinity(0))
+    warmup, reps = 16, 32
     for threads in (1, ncores):
-        assert ol.cpu_port_dot_product_matrix_8d(backend, rows, L, M, coeffs, srcs, threads) == 0
+        for _ in range(warmup):
+            assert ol.cpu_port_dot_product_matrix_8d(backend, rows, L, M, coeffs, srcs, threads) == 0
         samples = []
         t_end = time.perf_counter() + seconds_budget / 2
         while len(samples) < 64 and (time.perf_counter() < t_end or len(samples) < 3):
             t0 = time.perf_counter()
-            ol.cpu_port_dot_product_matrix_8d(backend, rows, L, M, coeffs, srcs, threads)
-            samples.append(time.perf_counter() - t0)
+            for _ in range(reps):
+                ol.cpu_port_dot_product_matrix_8d(backend, rows, L, M, coeffs, srcs, threads)
+            samples.append((time.perf_counter() - t0) / reps)
         med = float(np.median(samples))
         results[threads] = ((K + M) * L / med / 1e9, K * L / med / 2**30)
     # bit-exact check of the timed output against the scalar oracle on a slice
@@ -96,7 +100,8 @@ def cpu_baseline(seconds_budget: float = 12.0) -> dict:
         "unit": "GB/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"RS(10+4) Gf8D encode, 10 x 16 MiB shards, median of repeated runs; "
+        "sample": f"RS(10+4) Gf8D encode, 10 x 16 MiB shards, median of samples of 32 calls "
+                  f"after 16 warm-up calls (about 12 s of CPU work); "
                   f"C port of the reference's {name} path (oracle/cpu_port.c)",
         "all_cores": {"cores": ncores, "value": round(results[ncores][0], 2), "unit": "GB/s"},
         "source_GiBps_1core": round(results[1][1], 2),
@@ -280,6 +285,8 @@ def bench_configs(seed: int = 31) -> dict:
     * k = 64 survivors x 4 MiB rebuilding 16 erased slots of an 80-shard
       codeword through mul_add_matrix_scattered (decode matrix from the
       host inverse of the surviving 64 x 64 generator rows);
+    * RS(10+4) in the reed-solomon-erasure layout with 64 MiB shards: two
+      data and two parity shards rebuilt in one pass (checked by round trip);
     * k = 32, m = 8, 16 MiB per source for every tower field, uniform
       nonzero coefficients (VALU-bound: m*B byte products per source byte);
     * mul_elementwise on 2^26 uniform Gf64 pairs.
@@ -342,6 +349,27 @@ def bench_configs(seed: int = 31) -> dict:
     legs.append(_leg("Gf8D k=64 m=16 scattered reconstruct, 4 MiB shards", _median_launch_ms(run),
                      (k + 2 * m) * L, erased=erased))
     del region, saved
+
+    # RS(10+4) in the reed-solomon-erasure / klauspost layout over 64 MiB
+    # shards in separate allocations: two data and two parity shards rebuilt
+    # in one overwriting scattered pass (reads 10 survivors, writes 4).
+    from fgf_amd import interop
+    k, m, L = 10, 4, 64 << 20
+    rs = interop.ReedSolomon(k, m, "vandermonde")
+    shards = [rnd(L) for _ in range(k + m)]
+    rs.encode(shards)
+    lost = [1, 7, 10, 13]
+    saved = [shards[i].clone() for i in lost]
+    for i in lost:
+        shards[i].zero_()
+    present = [i not in lost for i in range(k + m)]
+    run = lambda: rs.reconstruct(shards, present)
+    run()
+    assert all(torch.equal(shards[i], s) for i, s in zip(lost, saved)), "interop reconstruct mismatch"
+    assert rs.verify(shards)
+    legs.append(_leg("Gf8D RS(10+4) reed-solomon-erasure layout, rebuild 2 data + 2 parity shards, "
+                     "64 MiB shards", _median_launch_ms(run), (k + len(lost)) * L, lost=lost))
+    del shards, saved
 
     # Towers: k = 32, m = 8, 16 MiB per source.
     k, m, L = 32, 8, 16 << 20

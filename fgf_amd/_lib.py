@@ -69,6 +69,9 @@ PROTOTYPES = {
                                              c_sizes, c_ptrs, c_sizes, c_size, c_vp]),
     "fgf_mul_add_matrix_scattered_with": (c_int, [c_int, c_u8p, c_size, c_size, c_sizes, c_size,
                                                   c_plan, c_ptrs, c_sizes, c_size, c_vp]),
+    "fgf_dot_product_matrix_scattered_with": (c_int, [c_int, c_u8p, c_size, c_size, c_sizes,
+                                                      c_size, c_plan, c_ptrs, c_sizes, c_size,
+                                                      c_vp]),
     "fgf_mul_elementwise": (c_int, [c_int, c_u8p, c_size, c_u8p, c_size, c_u8p, c_size, c_vp]),
     "fgf_coeff_table": (c_int, [c_int, c_u64, ctypes.POINTER(ctypes.c_uint32)]),
     "fgf_batch_create": (c_int, [c_int, c_vp, c_size, ctypes.POINTER(c_vp)]),

@@ -25,7 +25,7 @@ __all__ = [
     "dot_product_with", "mul_add_matrix", "mul_add_matrix_with", "dot_product_matrix",
     "dot_product_matrix_with", "dot_product_matrix_with_host", "mul_add_matrix_with_host",
     "mul_add_matrix_scattered",
-    "mul_add_matrix_scattered_with",
+    "mul_add_matrix_scattered_with", "dot_product_matrix_scattered_with",
     "mul_elementwise", "pack", "unpack", "pack_to_vec", "Panic",
 ]
 
@@ -388,6 +388,20 @@ def mul_add_matrix_scattered_with(field: Field, dst, row_len: int, row_starts: S
     _check(_lib.lib().fgf_mul_add_matrix_scattered_with(field.id, d, dl, row_len, starts,
                                                         len(row_starts), plan._h, ptrs, lens,
                                                         len(srcs), _stream(stream, dst, *srcs)))
+
+
+def dot_product_matrix_scattered_with(field: Field, dst, row_len: int, row_starts: Sequence[int],
+                                      plan: Plan, srcs: Sequence, stream=None) -> None:
+    """Overwriting scattered form (no counterpart in the reference): row j at
+    dst[row_starts[j]:][:row_len] becomes sum_t c[t][j] * srcs[t], for a plan
+    of dimensions (len(srcs), len(row_starts)); with no sources the rows
+    become zero."""
+    d, dl = _buf(dst)
+    starts = (ctypes.c_size_t * max(len(row_starts), 1))(*row_starts)
+    ptrs, lens = _ptr_array(srcs)
+    _check(_lib.lib().fgf_dot_product_matrix_scattered_with(
+        field.id, d, dl, row_len, starts, len(row_starts), plan._h, ptrs, lens, len(srcs),
+        _stream(stream, dst, *srcs)))
 
 
 def mul_elementwise(field: Field, dst, a, b, stream=None) -> None:

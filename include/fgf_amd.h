@@ -211,6 +211,15 @@ int fgf_mul_add_matrix_scattered_with(fgf_field field, uint8_t *dst, size_t dst_
                                       size_t row_len, const size_t *row_starts, size_t nrows,
                                       const fgf_plan *plan, const uint8_t *const *srcs,
                                       const size_t *src_lens, size_t nsrcs, void *stream);
+/* Overwriting form of the scattered shape: no counterpart in the reference.
+ * Row j at dst + row_starts[j] becomes sum_t c[t][j] * src[t] (zero with no
+ * sources), so a codec rebuilds erased shards in place in one pass without
+ * clearing them first. Checks and messages as the scattered forms above;
+ * nrows == 0 is a no-op. */
+int fgf_dot_product_matrix_scattered_with(fgf_field field, uint8_t *dst, size_t dst_len,
+                                          size_t row_len, const size_t *row_starts, size_t nrows,
+                                          const fgf_plan *plan, const uint8_t *const *srcs,
+                                          const size_t *src_lens, size_t nsrcs, void *stream);
 /* mul_elementwise (src/ops.rs:953-957): dst[i] = a[i] * b[i]. */
 int fgf_mul_elementwise(fgf_field field, uint8_t *dst, size_t dst_len, const uint8_t *a,
                         size_t a_len, const uint8_t *b, size_t b_len, void *stream);

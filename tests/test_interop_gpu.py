@@ -1,0 +1,196 @@
+"""GPU parity of the interoperable codec (fgf_amd.interop) and of the
+overwriting scattered entry point it runs on, bit-exact against the oracle.
+
+The parity a layout writes is checked against the oracle's matrix product
+with the same generator, and the Vandermonde case also against the
+published 4+2 matrix held in tests/golden/kats.json, so the device bytes are
+tied to what reed-solomon-erasure / klauspost / JavaReedSolomon write.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import oracle_lib as ol
+from fgf_amd import Gf8D, Gf16, interop, ops
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "kats.json")))
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    if not torch.cuda.is_available():
+        pytest.skip("no gfx950 device")
+    torch.cuda.init()
+
+
+def dev(a: np.ndarray) -> torch.Tensor:
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def host(t: torch.Tensor) -> np.ndarray:
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def oracle_parity(g, k, m, data):
+    L = len(data[0])
+    rows = np.zeros(m * L, np.uint8)
+    coeffs = [g[k + j][t] for t in range(k) for j in range(m)]
+    ol.mul_add_matrix("gf8d", rows, L, m, coeffs, data, True)
+    return rows.reshape(m, L)
+
+
+def shard_set(k, m, L, seed):
+    """Separate allocations, data then parity; parity starts as noise."""
+    data = [ol.noise(L, seed + i) for i in range(k)]
+    shards = [dev(d) for d in data] + [dev(ol.noise(L, seed + 100 + j)) for j in range(m)]
+    return data, shards
+
+
+@pytest.mark.parametrize("layout", interop.LAYOUTS)
+@pytest.mark.parametrize("k,m,L", [(4, 2, 4096), (10, 4, 65536 + 48), (17, 3, 1001), (1, 5, 64),
+                                   (32, 8, 1 << 20)])
+def test_encode_matches_the_oracle(layout, k, m, L):
+    print(f"\nencode {layout} k={k} m={m} L={L}", flush=True)
+    rs = interop.ReedSolomon(k, m, layout)
+    data, shards = shard_set(k, m, L, 7 * k + m)
+    rs.encode(shards)
+    want = oracle_parity(rs.matrix, k, m, data)
+    for j in range(m):
+        assert np.array_equal(host(shards[k + j]), want[j]), (layout, j)
+    for i in range(k):
+        assert np.array_equal(host(shards[i]), data[i])
+    assert rs.verify(shards)
+
+
+def test_vandermonde_parity_uses_the_published_matrix():
+    """Parity bytes from the device equal sum_c G[r][c] * data[c] with G the
+    published 4+2 matrix, evaluated by the oracle's byte products."""
+    k, m, L = 4, 2, 3000
+    rs = interop.ReedSolomon(k, m)
+    data, shards = shard_set(k, m, L, 42)
+    rs.encode(shards)
+    pub = GOLDEN["interop"]["vandermonde_4_2_parity_rows"]
+    for j in range(m):
+        want = np.zeros(L, np.uint8)
+        for c in range(k):
+            ol.mul_add("gf8d", want, pub[j][c], data[c])
+        assert np.array_equal(host(shards[k + j]), want)
+
+
+def test_encode_into_one_contiguous_codeword_region():
+    k, m, L = 10, 4, 1 << 16
+    rs = interop.ReedSolomon(k, m)
+    region = dev(ol.noise((k + m) * L, 5))
+    shards = list(region.view(k + m, L))
+    data = [host(s).copy() for s in shards[:k]]
+    rs.encode(shards)
+    want = oracle_parity(rs.matrix, k, m, data)
+    assert np.array_equal(host(region).reshape(k + m, L)[k:], want)
+
+
+@pytest.mark.parametrize("layout", ["vandermonde", "cauchy"])
+@pytest.mark.parametrize("k,m", [(4, 2), (10, 4), (20, 6)])
+def test_reconstruct_every_pattern_class(layout, k, m):
+    L = 8192 + 16
+    rs = interop.ReedSolomon(k, m, layout)
+    data, shards = shard_set(k, m, L, 3 * k)
+    rs.encode(shards)
+    original = [host(s).copy() for s in shards]
+    rng = np.random.default_rng(k * m)
+    patterns = [list(range(m)),                                   # data shards only
+                list(range(k, k + m)),                            # parity only
+                sorted(rng.choice(k + m, m, replace=False).tolist()),
+                sorted(rng.choice(k + m, max(1, m // 2), replace=False).tolist())]
+    for lost in patterns:
+        print(f"\nreconstruct {layout} k={k} m={m} lost={lost}", flush=True)
+        for i in lost:
+            shards[i].fill_(0xA5)
+        present = [i not in lost for i in range(k + m)]
+        rs.reconstruct(shards, present)
+        for i in range(k + m):
+            assert np.array_equal(host(shards[i]), original[i]), (lost, i)
+
+
+def test_reconstruct_data_leaves_missing_parity_alone():
+    k, m, L = 6, 3, 4096
+    rs = interop.ReedSolomon(k, m)
+    data, shards = shard_set(k, m, L, 11)
+    rs.encode(shards)
+    original = [host(s).copy() for s in shards]
+    lost = [0, 4, 7]
+    for i in lost:
+        shards[i].fill_(0x5A)
+    rs.reconstruct_data(shards, [i not in lost for i in range(k + m)])
+    assert np.array_equal(host(shards[0]), original[0])
+    assert np.array_equal(host(shards[4]), original[4])
+    assert (host(shards[7]) == 0x5A).all()
+
+
+def test_verify_detects_a_flipped_byte_and_arguments_are_checked():
+    k, m, L = 5, 2, 512
+    rs = interop.ReedSolomon(k, m)
+    _, shards = shard_set(k, m, L, 13)
+    rs.encode(shards)
+    assert rs.verify(shards)
+    shards[k + 1][100] ^= 1
+    assert not rs.verify(shards)
+    with pytest.raises(ValueError, match="too few shards present"):
+        rs.reconstruct(shards, [True] * (k - 1) + [False] * (m + 1))
+    with pytest.raises(ValueError, match="too few shards"):
+        rs.encode(shards[:-1])
+    with pytest.raises(ValueError, match="incorrect shard size"):
+        rs.encode(shards[:-1] + [torch.zeros(L + 1, dtype=torch.uint8, device="cuda")])
+    with pytest.raises(ValueError, match="empty shard"):
+        rs.encode([torch.zeros(0, dtype=torch.uint8, device="cuda")] * (k + m))
+
+
+def test_isal_rs_singular_set_is_refused():
+    k, m = 5, 6
+    rs = interop.ReedSolomon(k, m, "isal_rs")
+    _, shards = shard_set(k, m, 256, 17)
+    rs.encode(shards)
+    have = {1, 2, 5, 7, 10}
+    with pytest.raises(ValueError, match="singular"):
+        rs.reconstruct(shards, [i in have for i in range(k + m)])
+
+
+@pytest.mark.parametrize("f", [Gf8D, Gf16], ids=["gf8d", "gf16"])
+def test_overwriting_scattered_rows(f):
+    """dot_product_matrix_scattered_with writes sum_t c[t][j] * src[t] at each
+    row start, leaves the gaps alone, and zeroes the rows with no sources."""
+    name = {Gf8D.id: "gf8d", Gf16.id: "gf16"}[f.id]
+    k, L = 7, 4096 + 64
+    starts = [3 * L + 8, 0, L + 40]                 # unordered, gaps between rows
+    m = len(starts)
+    size = 4 * L + 64
+    init = ol.noise(size, 1)
+    srcs = [ol.noise(L, 10 + t) for t in range(k)]
+    b = ol.noise(k * m * f.BYTES, 99)
+    coeffs = [int.from_bytes(b[i * f.BYTES:(i + 1) * f.BYTES].tobytes(), "little")
+              for i in range(k * m)]
+    plan = ops.Plan.matrix(f, k, m, coeffs)
+    dst = dev(init)
+    ops.dot_product_matrix_scattered_with(f, dst, L, starts, plan, [dev(s) for s in srcs])
+    want = init.copy()
+    for s in starts:
+        want[s:s + L] = 0
+    ol.mul_add_matrix_scattered(name, want, L, starts, coeffs, srcs)
+    assert np.array_equal(host(dst), want)
+    # no sources: the rows become zero, the gaps keep their bytes
+    dst = dev(init)
+    ops.dot_product_matrix_scattered_with(f, dst, L, starts, ops.Plan.matrix(f, 0, m, []), [])
+    want = init.copy()
+    for s in starts:
+        want[s:s + L] = 0
+    assert np.array_equal(host(dst), want)
+    # nrows == 0 is a no-op; overlapping rows are refused
+    ops.dot_product_matrix_scattered_with(f, dst, L, [], ops.Plan.matrix(f, k, 0, []),
+                                          [dev(s) for s in srcs])
+    with pytest.raises(ops.Panic, match="overlap"):
+        ops.dot_product_matrix_scattered_with(f, dst, L, [0, L // 2], ops.Plan.matrix(f, 0, 2, []), [])
