@@ -416,6 +416,20 @@ void mul_add_matrix_scattered(Bytes dst, std::size_t row_len,
                                                terms.size(), stream));
 }
 
+/* Overwriting scattered form with a (sources, rows) plan; no counterpart in
+ * the reference: row j at dst + row_starts[j] becomes sum_t c[t][j] * src[t]. */
+template <class F>
+void dot_product_matrix_scattered_with(Bytes dst, std::size_t row_len,
+                                       const std::vector<std::size_t> &row_starts,
+                                       const Plan<F> &plan, const std::vector<CBytes> &srcs,
+                                       void *stream = nullptr) {
+    detail::Srcs s(srcs);
+    detail::check(fgf_dot_product_matrix_scattered_with(F::ID, dst.ptr, dst.len, row_len,
+                                                        row_starts.data(), row_starts.size(),
+                                                        plan.raw(), s.ptrs.data(), s.lens.data(),
+                                                        srcs.size(), stream));
+}
+
 /* mul_elementwise (src/ops.rs:953-957): dst[i] = a[i] * b[i]. */
 template <class F>
 void mul_elementwise(Bytes dst, CBytes a, CBytes b, void *stream = nullptr) {

@@ -356,6 +356,39 @@ void check_matrix_scattered(uint64_t seed) {
                                     sp.data(), k);
     REQUIRE(d.get() == want, "scattered");
 }
+// The overwriting scattered form (no counterpart in the reference): the rows
+// equal the accumulate form applied to zeroed rows; the gaps keep their bytes.
+template <class F>
+void check_dot_product_matrix_scattered(uint64_t seed) {
+    const size_t row_len = 130 * F::BYTES, nrows = 3, k = 5;
+    const std::vector<size_t> starts = {4 * row_len + 2 * F::BYTES, 0, row_len + 6 * F::BYTES};
+    std::vector<Vec> srcs;
+    std::vector<std::unique_ptr<Dev>> ds;
+    std::vector<CBytes> views;
+    std::vector<typename F::Elem> flat;
+    for (size_t t = 0; t < k; ++t) {
+        srcs.push_back(noise(row_len, seed + t));
+        ds.push_back(std::make_unique<Dev>(srcs.back()));
+        views.push_back(ds.back()->c());
+        auto col = rand_elems<F>(nrows, seed + 10 + t);
+        flat.insert(flat.end(), col.begin(), col.end());
+    }
+    const Vec init = noise(6 * row_len, seed + 20);
+    Dev d(init);
+    ops::dot_product_matrix_scattered_with<F>(d.b(), row_len, starts, Plan<F>::matrix(k, nrows, flat),
+                                              views);
+    Vec want = init;
+    for (size_t s : starts) std::fill(want.begin() + s, want.begin() + s + row_len, 0);
+    std::vector<uint64_t> c64(flat.begin(), flat.end());
+    std::vector<const uint8_t *> sp;
+    for (const Vec &s : srcs) sp.push_back(s.data());
+    oracle_mul_add_matrix_scattered(F::ID, want.data(), row_len, starts.data(), nrows, c64.data(),
+                                    sp.data(), k);
+    REQUIRE(d.get() == want, "overwriting scattered");
+}
+TEST(gf8d_dot_product_matrix_scattered_overwrites) { check_dot_product_matrix_scattered<Gf8D>(0x83); }
+TEST(gf32_dot_product_matrix_scattered_overwrites) { check_dot_product_matrix_scattered<Gf32>(0x84); }
+
 TEST(gf8_matrix_scattered_matches_contiguous) { check_matrix_scattered<Gf8B>(0x80); }
 TEST(gf16_matrix_scattered_matches_contiguous) { check_matrix_scattered<Gf16>(0x81); }
 TEST(gf8d_matrix_scattered_matches_contiguous) { check_matrix_scattered<Gf8D>(0x82); }

@@ -4,7 +4,9 @@ overwriting scattered entry point it runs on, bit-exact against the oracle.
 The parity a layout writes is checked against the oracle's matrix product
 with the same generator, and the Vandermonde case also against the
 published 4+2 matrix held in tests/golden/kats.json, so the device bytes are
-tied to what reed-solomon-erasure / klauspost / JavaReedSolomon write.
+tied to what reed-solomon-erasure / klauspost / JavaReedSolomon write. The
+reference's own cross-checks against reed-solomon-erasure
+(benches/compare.rs) run here with that bench's exact inputs.
 """
 import json
 import os
@@ -14,7 +16,7 @@ import pytest
 import torch
 
 import oracle_lib as ol
-from fgf_amd import Gf8D, Gf16, interop, ops
+from fgf_amd import Gf8B, Gf8D, Gf16, interop, ops
 
 pytestmark = pytest.mark.gpu
 
@@ -194,3 +196,50 @@ def test_overwriting_scattered_rows(f):
                                           [dev(s) for s in srcs])
     with pytest.raises(ops.Panic, match="overlap"):
         ops.dot_product_matrix_scattered_with(f, dst, L, [0, L // 2], ops.Plan.matrix(f, 0, 2, []), [])
+
+
+# ---------------------------------------------------------------------------
+# The reference's own cross-checks against reed-solomon-erasure
+# (benches/compare.rs), with the bench's exact inputs.
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("length", [4 * 1024, 16 * 1024])          # compare.rs:25
+def test_compare_bench_dot_product_equals_summed_mul_slice_xor(length):
+    """compare.rs:82-107: 16 sources noise(len, 0x800 + i), coefficient bytes
+    2 + (i * 73 + 19) % 254; Gf8D dot_product_with must equal zeroing the
+    destination and applying reed-solomon-erasure's galois_8::mul_slice_xor
+    (dst ^= c * src over 0x11D) once per source, which the oracle's mul_add
+    restates."""
+    n = 16
+    srcs = [ol.noise(length, 0x800 + i) for i in range(n)]
+    coeffs = [2 + (i * 73 + 19) % 254 for i in range(n)]
+    plan = ops.Plan.new(Gf8D, coeffs)
+    dst = dev(ol.noise(length, 0x901))
+    ops.dot_product_with(Gf8D, dst, plan, [dev(s) for s in srcs])
+    want = np.zeros(length, np.uint8)
+    for c, s in zip(coeffs, srcs):
+        ol.mul_add("gf8d", want, c, s)
+    assert np.array_equal(host(dst), want)
+
+
+@pytest.mark.parametrize("nrows", [2, 4, 6])                       # compare.rs:27
+def test_compare_bench_matrix_overwrite_equals_fill_and_mul_add(nrows):
+    """compare.rs:129-178: 10 sources noise(64 KiB, 0xa00 + term), coefficient
+    (row, term) = 1 + ((row * 10 + term) * 97 + 13) % 255; dot_product_matrix
+    must equal fill(0) + mul_add_matrix, for Gf8B and Gf8D."""
+    k, L = 10, 64 * 1024
+    srcs = [dev(ol.noise(L, 0xa00 + t)) for t in range(k)]
+    for f, seeds in ((Gf8B, (0xb00, 0xb01)), (Gf8D, (0xb02, 0xb03))):
+        cols = [[1 + ((row * k + t) * 97 + 13) % 255 for row in range(nrows)] for t in range(k)]
+        terms = list(zip(cols, srcs))
+        old = dev(ol.noise(L * nrows, seeds[0]))
+        new = dev(ol.noise(L * nrows, seeds[1]))
+        old.zero_()
+        ops.mul_add_matrix(f, old, L, nrows, terms)
+        ops.dot_product_matrix(f, new, L, nrows, terms)
+        assert torch.equal(old, new)
+        name = "gf8b" if f.id == 0 else "gf8d"
+        want = np.zeros(L * nrows, np.uint8)
+        flat = [cols[t][j] for t in range(k) for j in range(nrows)]
+        ol.mul_add_matrix(name, want, L, nrows, flat, [host(s) for s in srcs], True)
+        assert np.array_equal(host(new), want)
