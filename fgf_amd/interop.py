@@ -29,11 +29,12 @@ rebuilds every missing shard with one overwriting scattered launch.
 """
 from __future__ import annotations
 
+import bisect
 from collections import OrderedDict
 from typing import Sequence
 
-from . import codec, ops
-from .field import Gf8D, Panic
+from . import _lib, codec, ops
+from .field import Gf8D, Panic, _check
 
 LAYOUTS = ("vandermonde", "cauchy", "isal_rs")
 MAX_SHARDS = 256
@@ -90,6 +91,21 @@ def generator(k: int, m: int, layout: str = "vandermonde") -> list[list[int]]:
     return _BUILDERS[layout](k, m)
 
 
+def _check_disjoint(dst: Sequence, src: Sequence) -> None:
+    """Written shards must not overlap each other or a shard that is read
+    (reed-solomon-erasure takes them as distinct `&mut` buffers)."""
+    spans = sorted((t.data_ptr(), t.data_ptr() + t.nbytes) for t in dst)
+    for (_, end), (start, _) in zip(spans, spans[1:]):
+        if start < end:
+            raise ValueError("shards overlap: every written shard needs its own bytes")
+    starts = [a for a, _ in spans]
+    for t in src:
+        a, b = t.data_ptr(), t.data_ptr() + t.nbytes
+        i = bisect.bisect_right(starts, a)
+        if (i > 0 and spans[i - 1][1] > a) or (i < len(spans) and spans[i][0] < b):
+            raise ValueError("shards overlap: a shard that is read is also written")
+
+
 def _region(tensors: Sequence) -> tuple[ops.Buf, list[int]]:
     """One byte range spanning `tensors` and each tensor's offset in it. The
     kernels touch only the tensors' own bytes, never the gaps between them."""
@@ -144,6 +160,7 @@ class ReedSolomon:
         if len(data) != k or len(parity) != m:
             raise ValueError(f"expected {k} data and {m} parity shards, got {len(data)} and {len(parity)}")
         n = self._shard_len(list(data) + list(parity))
+        _check_disjoint(parity, data)
         dst, starts = _region(parity)
         ops.dot_product_matrix_scattered_with(Gf8D, dst, n, starts, self._encode, list(data), stream)
 
@@ -154,7 +171,11 @@ class ReedSolomon:
         k, m = self.data_shards, self.parity_shards
         fresh = torch.empty((m, n), dtype=torch.uint8, device=shards[0].device)
         ops.dot_product_matrix_with(Gf8D, fresh, n, m, self._encode, list(shards[:k]), stream)
-        return all(torch.equal(fresh[j], shards[k + j].view(-1)) for j in range(m))
+        if stream is not None:
+            # The comparison runs on torch's current stream.
+            _check(_lib.lib().fgf_stream_synchronize(int(stream)))
+        return all(torch.equal(fresh[j], shards[k + j].reshape(-1).view(torch.uint8))
+                   for j in range(m))
 
     def _decode_plan(self, survivors: tuple, rebuild: tuple) -> ops.Plan:
         key = (survivors, rebuild)
@@ -188,6 +209,7 @@ class ReedSolomon:
         if not missing:
             return
         survivors = tuple(have[:k])
+        _check_disjoint([shards[i] for i in missing], [shards[i] for i in survivors])
         plan = self._decode_plan(survivors, missing)
         dst, starts = _region([shards[i] for i in missing])
         ops.dot_product_matrix_scattered_with(Gf8D, dst, n, starts, plan,

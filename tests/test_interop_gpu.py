@@ -150,6 +150,17 @@ def test_verify_detects_a_flipped_byte_and_arguments_are_checked():
         rs.encode(shards[:-1] + [torch.zeros(L + 1, dtype=torch.uint8, device="cuda")])
     with pytest.raises(ValueError, match="empty shard"):
         rs.encode([torch.zeros(0, dtype=torch.uint8, device="cuda")] * (k + m))
+    with pytest.raises(ValueError, match="shards overlap"):
+        rs.encode(shards[:k] + [shards[k], shards[k]])                # parity twice
+    with pytest.raises(ValueError, match="also written"):
+        rs.encode(shards[:k] + [shards[0], shards[k + 1]])            # parity aliases data
+    region = torch.zeros(2 * L, dtype=torch.uint8, device="cuda")
+    with pytest.raises(ValueError, match="also written"):                # partial overlap
+        rs.encode([region[:L]] + shards[1:k] + [region[L // 2:L // 2 + L], shards[k + 1]])
+    # a caller's stream: verify waits for it before comparing
+    s = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    assert not rs.verify(shards, stream=s.cuda_stream)
 
 
 def test_isal_rs_singular_set_is_refused():
