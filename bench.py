@@ -106,7 +106,19 @@ def cpu_baseline(seconds_budget: float = 16.0) -> dict:
         "all_cores": {"cores": ncores, "value": round(results[ncores][0], 2), "unit": "GB/s"},
         "source_GiBps_1core": round(results[1][1], 2),
         "backend": name,
+        "gfni": backend == 0,
+        "cpu_model": _cpu_model(),
     }
+
+
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def bench_batched(n_stripes: int = 4096, seed: int = 4096, reps: int = 5) -> dict:
