@@ -968,34 +968,15 @@ int fgf_mul_add_matrix_scattered(fgf_field f, uint8_t *dst, size_t dst_len, size
                       S(stream));
 }
 
-int fgf_mul_add_matrix_scattered_with(fgf_field f, uint8_t *dst, size_t dst_len, size_t row_len,
-                                      const size_t *row_starts, size_t nrows, const fgf_plan *plan,
-                                      const uint8_t *const *srcs, const size_t *src_lens,
-                                      size_t nsrcs, void *stream) {
-    FIELD_GUARD(f);
-    const char *name = "mul_add_matrix_scattered_with";
-    int rc = 0;
-    if (!check_scattered_rows(f, name, dst_len, row_len, row_starts, nrows, &rc)) return rc;
-    if (!plan_ok(f, plan, name, &rc)) return rc;
-    if (plan->rows != nsrcs || plan->cols != nrows)
-        return fail("%s: plan dimensions do not match (sources, rows)", name);
-    for (size_t i = 0; i < nsrcs; ++i)
-        if (src_lens[i] != row_len)
-            return fail("%s: source %zu is %zu bytes, expected %zu", name, i, src_lens[i], row_len);
-    if (nrows == 0 || nsrcs == 0 || row_len == 0) return FGF_OK;
-    std::vector<uint8_t *> r(nrows);
-    for (size_t j = 0; j < nrows; ++j) r[j] = dst + row_starts[j];
-    const size_t e = entry_dw(plan->nb);
-    return run_matrix(f, plan->d_tab, nrows * e, e, r.data(), nrows, srcs, nsrcs, row_len, false,
-                      S(stream));
-}
+namespace {
 
-int fgf_dot_product_matrix_scattered_with(fgf_field f, uint8_t *dst, size_t dst_len, size_t row_len,
-                                          const size_t *row_starts, size_t nrows, const fgf_plan *plan,
-                                          const uint8_t *const *srcs, const size_t *src_lens,
-                                          size_t nsrcs, void *stream) {
-    FIELD_GUARD(f);
-    const char *name = "dot_product_matrix_scattered_with";
+// The prepared scattered forms: accumulate (mul_add_matrix_scattered_with) or
+// overwrite (dot_product_matrix_scattered_with, whose rows become zero with
+// no sources). Same checks and messages for both.
+int scattered_with(int f, const char *name, uint8_t *dst, size_t dst_len, size_t row_len,
+                   const size_t *row_starts, size_t nrows, const fgf_plan *plan,
+                   const uint8_t *const *srcs, const size_t *src_lens, size_t nsrcs,
+                   bool overwrite, hipStream_t s) {
     int rc = 0;
     if (!check_scattered_rows(f, name, dst_len, row_len, row_starts, nrows, &rc)) return rc;
     if (!plan_ok(f, plan, name, &rc)) return rc;
@@ -1006,16 +987,35 @@ int fgf_dot_product_matrix_scattered_with(fgf_field f, uint8_t *dst, size_t dst_
             return fail("%s: source %zu is %zu bytes, expected %zu", name, i, src_lens[i], row_len);
     if (nrows == 0 || row_len == 0) return FGF_OK;
     if (nsrcs == 0) {
-        // No terms: the rows become zero, as dot_product_matrix zeroes rows[..used].
+        if (!overwrite) return FGF_OK;
         for (size_t j = 0; j < nrows; ++j)
-            if ((rc = fill_zero(dst + row_starts[j], row_len, S(stream)))) return rc;
+            if ((rc = fill_zero(dst + row_starts[j], row_len, s))) return rc;
         return FGF_OK;
     }
     std::vector<uint8_t *> r(nrows);
     for (size_t j = 0; j < nrows; ++j) r[j] = dst + row_starts[j];
     const size_t e = entry_dw(plan->nb);
-    return run_matrix(f, plan->d_tab, nrows * e, e, r.data(), nrows, srcs, nsrcs, row_len, true,
-                      S(stream));
+    return run_matrix(f, plan->d_tab, nrows * e, e, r.data(), nrows, srcs, nsrcs, row_len, overwrite, s);
+}
+
+}  // namespace
+
+int fgf_mul_add_matrix_scattered_with(fgf_field f, uint8_t *dst, size_t dst_len, size_t row_len,
+                                      const size_t *row_starts, size_t nrows, const fgf_plan *plan,
+                                      const uint8_t *const *srcs, const size_t *src_lens,
+                                      size_t nsrcs, void *stream) {
+    FIELD_GUARD(f);
+    return scattered_with(f, "mul_add_matrix_scattered_with", dst, dst_len, row_len, row_starts, nrows,
+                          plan, srcs, src_lens, nsrcs, false, S(stream));
+}
+
+int fgf_dot_product_matrix_scattered_with(fgf_field f, uint8_t *dst, size_t dst_len, size_t row_len,
+                                          const size_t *row_starts, size_t nrows, const fgf_plan *plan,
+                                          const uint8_t *const *srcs, const size_t *src_lens,
+                                          size_t nsrcs, void *stream) {
+    FIELD_GUARD(f);
+    return scattered_with(f, "dot_product_matrix_scattered_with", dst, dst_len, row_len, row_starts,
+                          nrows, plan, srcs, src_lens, nsrcs, true, S(stream));
 }
 
 int fgf_mul_elementwise(fgf_field f, uint8_t *dst, size_t dst_len, const uint8_t *a, size_t a_len,
