@@ -96,6 +96,27 @@ def test_encode_into_one_contiguous_codeword_region():
     assert np.array_equal(host(region).reshape(k + m, L)[k:], want)
 
 
+def test_misaligned_shard_views_encode_and_rebuild():
+    """Shards at odd byte offsets of one buffer (the unaligned kernel path)."""
+    k, m, L = 6, 3, 1000
+    rs = interop.ReedSolomon(k, m)
+    region = dev(ol.noise((k + m) * L + 3, 21))
+    shards = [region[3 + i * L:3 + (i + 1) * L] for i in range(k + m)]
+    data = [host(s).copy() for s in shards[:k]]
+    rs.encode(shards)
+    want = oracle_parity(rs.matrix, k, m, data)
+    for j in range(m):
+        assert np.array_equal(host(shards[k + j]), want[j])
+    assert host(region)[:3].tobytes() == ol.noise(3, 21).tobytes()
+    original = [host(s).copy() for s in shards]
+    lost = [0, 5, 7]
+    for i in lost:
+        shards[i].fill_(0)
+    rs.reconstruct(shards, [i not in lost for i in range(k + m)])
+    for i in range(k + m):
+        assert np.array_equal(host(shards[i]), original[i]), i
+
+
 @pytest.mark.parametrize("layout", ["vandermonde", "cauchy"])
 @pytest.mark.parametrize("k,m", [(4, 2), (10, 4), (20, 6)])
 def test_reconstruct_every_pattern_class(layout, k, m):
