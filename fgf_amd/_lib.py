@@ -61,6 +61,12 @@ PROTOTYPES = {
     "fgf_dot_product_matrix_with": (c_int, [c_int, c_u8p, c_size, c_size, c_size, c_plan, c_ptrs,
                                             c_sizes, c_size, c_vp]),
     "fgf_stream_synchronize": (c_int, [c_vp]),
+    "fgf_mul_add_matrix_scattered_with_host": (c_int, [c_int, c_u8p, c_size, c_size, c_sizes,
+                                                       c_size, c_plan, c_ptrs, c_sizes, c_size,
+                                                       c_vp]),
+    "fgf_dot_product_matrix_scattered_with_host": (c_int, [c_int, c_u8p, c_size, c_size, c_sizes,
+                                                           c_size, c_plan, c_ptrs, c_sizes, c_size,
+                                                           c_vp]),
     "fgf_mul_add_matrix_with_host": (c_int, [c_int, c_u8p, c_size, c_size, c_size, c_plan,
                                              c_ptrs, c_sizes, c_size, c_vp]),
     "fgf_dot_product_matrix_with_host": (c_int, [c_int, c_u8p, c_size, c_size, c_size, c_plan,

@@ -800,26 +800,24 @@ hipError_t pipe_reserve(HostPipe &p, size_t bytes) {
 
 namespace {
 
-// The host-buffer pipeline over column chunks. Accumulating (overwrite
-// false), each chunk of the destination rows goes up with its sources and
-// comes back down with the products folded in.
-int host_matrix(int f, const char *name, uint8_t *rows, size_t rows_len, size_t row_len,
-                size_t nrows, const fgf_plan *plan, const uint8_t *const *srcs,
-                const size_t *src_lens, size_t nsrcs, bool overwrite, hipStream_t s) {
+// The host-buffer pipeline over column chunks, for validated arguments: row
+// j lives at hrows[j] on the host. Accumulating (overwrite false), each chunk
+// of the destination rows goes up with its sources and comes back down with
+// the products folded in.
+int host_pipeline(int f, const char *name, const std::vector<uint8_t *> &hrows, size_t row_len,
+                  const fgf_plan *plan, const uint8_t *const *srcs, size_t nsrcs, bool overwrite,
+                  hipStream_t s) {
+    const size_t nrows = hrows.size();
     int rc = 0;
-    size_t used;
-    if (!check_matrix_plan(f, name, rows_len, row_len, nrows, plan, src_lens, nsrcs, &used, &rc))
-        return rc;
-    if (nrows == 0) return FGF_OK;
     hipError_t e;
+    if (nrows == 0 || row_len == 0) return FGF_OK;
     if (nsrcs == 0) {
         if (!overwrite) return FGF_OK;
-        // No terms: zero rows[..used] once earlier work on the stream is done.
+        // No terms: zero the rows once earlier work on the stream is done.
         if ((e = hipStreamSynchronize(s)) != hipSuccess) return device_fail("hipStreamSynchronize", e);
-        std::memset(rows, 0, used);
+        for (uint8_t *r : hrows) std::memset(r, 0, row_len);
         return FGF_OK;
     }
-    if (row_len == 0) return FGF_OK;
     int dev = 0;
     if ((e = hipGetDevice(&dev)) != hipSuccess) return device_fail("hipGetDevice", e);
     if (dev < 0 || dev >= HOST_MAX_DEVICES) return fail("%s: device %d", name, dev);
@@ -857,7 +855,7 @@ int host_matrix(int f, const char *name, uint8_t *rows, size_t rows_len, size_t 
                 return device_fail("hipMemcpyAsync H2D", e);
         if (!overwrite)
             for (size_t j = 0; j < m; ++j)
-                if ((e = hipMemcpyAsync(drow[j], rows + j * row_len + c0, c, hipMemcpyHostToDevice, up)) !=
+                if ((e = hipMemcpyAsync(drow[j], hrows[j] + c0, c, hipMemcpyHostToDevice, up)) !=
                     hipSuccess)
                     return device_fail("hipMemcpyAsync H2D", e);
         if ((e = hipEventRecord(p.ev[0][b], up)) != hipSuccess ||
@@ -869,7 +867,7 @@ int host_matrix(int f, const char *name, uint8_t *rows, size_t rows_len, size_t 
             (e = hipStreamWaitEvent(down, p.ev[1][b], 0)) != hipSuccess)
             return device_fail("host pipeline ordering", e);
         for (size_t j = 0; j < m; ++j)
-            if ((e = hipMemcpyAsync(rows + j * row_len + c0, drow[j], c, hipMemcpyDeviceToHost, down)) != hipSuccess)
+            if ((e = hipMemcpyAsync(hrows[j] + c0, drow[j], c, hipMemcpyDeviceToHost, down)) != hipSuccess)
                 return device_fail("hipMemcpyAsync D2H", e);
         if ((e = hipEventRecord(p.ev[2][b], down)) != hipSuccess) return device_fail("host pipeline ordering", e);
         p.used[b] = true;
@@ -877,6 +875,37 @@ int host_matrix(int f, const char *name, uint8_t *rows, size_t rows_len, size_t 
     if ((e = hipEventRecord(p.ev_out, down)) != hipSuccess || (e = hipStreamWaitEvent(s, p.ev_out, 0)) != hipSuccess)
         return device_fail("host pipeline ordering", e);
     return FGF_OK;
+}
+
+int host_matrix(int f, const char *name, uint8_t *rows, size_t rows_len, size_t row_len,
+                size_t nrows, const fgf_plan *plan, const uint8_t *const *srcs,
+                const size_t *src_lens, size_t nsrcs, bool overwrite, hipStream_t s) {
+    int rc = 0;
+    size_t used;
+    if (!check_matrix_plan(f, name, rows_len, row_len, nrows, plan, src_lens, nsrcs, &used, &rc))
+        return rc;
+    return host_pipeline(f, name, contiguous_rows(rows, nrows, row_len), row_len, plan, srcs, nsrcs,
+                         overwrite, s);
+}
+
+bool check_scattered_rows(int f, const char *name, size_t dst_len, size_t row_len,
+                          const size_t *row_starts, size_t nrows, int *rc);
+
+int host_scattered(int f, const char *name, uint8_t *dst, size_t dst_len, size_t row_len,
+                   const size_t *row_starts, size_t nrows, const fgf_plan *plan,
+                   const uint8_t *const *srcs, const size_t *src_lens, size_t nsrcs, bool overwrite,
+                   hipStream_t s) {
+    int rc = 0;
+    if (!check_scattered_rows(f, name, dst_len, row_len, row_starts, nrows, &rc)) return rc;
+    if (!plan_ok(f, plan, name, &rc)) return rc;
+    if (plan->rows != nsrcs || plan->cols != nrows)
+        return fail("%s: plan dimensions do not match (sources, rows)", name);
+    for (size_t i = 0; i < nsrcs; ++i)
+        if (src_lens[i] != row_len)
+            return fail("%s: source %zu is %zu bytes, expected %zu", name, i, src_lens[i], row_len);
+    std::vector<uint8_t *> r(nrows);
+    for (size_t j = 0; j < nrows; ++j) r[j] = dst + row_starts[j];
+    return host_pipeline(f, name, r, row_len, plan, srcs, nsrcs, overwrite, s);
 }
 
 }  // namespace
@@ -895,6 +924,24 @@ int fgf_mul_add_matrix_with_host(fgf_field f, uint8_t *rows, size_t rows_len, si
     FIELD_GUARD(f);
     return host_matrix(f, "mul_add_matrix_with_host", rows, rows_len, row_len, nrows, plan, srcs,
                        src_lens, nsrcs, false, S(stream));
+}
+
+int fgf_mul_add_matrix_scattered_with_host(fgf_field f, uint8_t *dst, size_t dst_len, size_t row_len,
+                                           const size_t *row_starts, size_t nrows, const fgf_plan *plan,
+                                           const uint8_t *const *srcs, const size_t *src_lens,
+                                           size_t nsrcs, void *stream) {
+    FIELD_GUARD(f);
+    return host_scattered(f, "mul_add_matrix_scattered_with_host", dst, dst_len, row_len, row_starts,
+                          nrows, plan, srcs, src_lens, nsrcs, false, S(stream));
+}
+
+int fgf_dot_product_matrix_scattered_with_host(fgf_field f, uint8_t *dst, size_t dst_len,
+                                               size_t row_len, const size_t *row_starts, size_t nrows,
+                                               const fgf_plan *plan, const uint8_t *const *srcs,
+                                               const size_t *src_lens, size_t nsrcs, void *stream) {
+    FIELD_GUARD(f);
+    return host_scattered(f, "dot_product_matrix_scattered_with_host", dst, dst_len, row_len, row_starts,
+                          nrows, plan, srcs, src_lens, nsrcs, true, S(stream));
 }
 
 int fgf_stream_synchronize(void *stream) {

@@ -22,10 +22,14 @@ Generator layouts ((k + m) x k, the top k rows the identity):
   matrix is not invertible for every erasure pattern once m is large.
 
 `ReedSolomon` mirrors reed-solomon-erasure's `ReedSolomon` (new, encode,
-encode_sep, verify, reconstruct, reconstruct_data) over device shards: a
-list of equal-length uint8 device tensors, data shards first. Reconstruction
-uses the first k present shards in index order, as that crate does, and
-rebuilds every missing shard with one overwriting scattered launch.
+encode_sep, verify, reconstruct, reconstruct_data) over a list of
+equal-length shards, data shards first: device tensors (the calls are
+asynchronous on the stream, like every op here), or host buffers (numpy
+arrays or CPU tensors, pinned for full PCIe overlap), which go through the
+library's host-buffer pipeline and are written when the call returns, as
+that crate's are. Reconstruction uses the first k present shards in index
+order, as that crate does, and rebuilds every missing shard in one
+overwriting scattered pass.
 """
 from __future__ import annotations
 
@@ -91,16 +95,31 @@ def generator(k: int, m: int, layout: str = "vandermonde") -> list[list[int]]:
     return _BUILDERS[layout](k, m)
 
 
+def _ptr(t) -> int:
+    return t.data_ptr() if hasattr(t, "data_ptr") else t.ctypes.data
+
+
+def _contiguous(t) -> bool:
+    return t.is_contiguous() if hasattr(t, "is_contiguous") else t.flags["C_CONTIGUOUS"]
+
+
+def _on_host(shards: Sequence) -> bool:
+    cuda = [bool(getattr(t, "is_cuda", False)) for t in shards]
+    if any(cuda) and not all(cuda):
+        raise ValueError("shards must all be device tensors or all host buffers")
+    return not cuda[0]
+
+
 def _check_disjoint(dst: Sequence, src: Sequence) -> None:
     """Written shards must not overlap each other or a shard that is read
     (reed-solomon-erasure takes them as distinct `&mut` buffers)."""
-    spans = sorted((t.data_ptr(), t.data_ptr() + t.nbytes) for t in dst)
+    spans = sorted((_ptr(t), _ptr(t) + t.nbytes) for t in dst)
     for (_, end), (start, _) in zip(spans, spans[1:]):
         if start < end:
             raise ValueError("shards overlap: every written shard needs its own bytes")
     starts = [a for a, _ in spans]
     for t in src:
-        a, b = t.data_ptr(), t.data_ptr() + t.nbytes
+        a, b = _ptr(t), _ptr(t) + t.nbytes
         i = bisect.bisect_right(starts, a)
         if (i > 0 and spans[i - 1][1] > a) or (i < len(spans) and spans[i][0] < b):
             raise ValueError("shards overlap: a shard that is read is also written")
@@ -109,15 +128,20 @@ def _check_disjoint(dst: Sequence, src: Sequence) -> None:
 def _region(tensors: Sequence) -> tuple[ops.Buf, list[int]]:
     """One byte range spanning `tensors` and each tensor's offset in it. The
     kernels touch only the tensors' own bytes, never the gaps between them."""
-    ptrs = [t.data_ptr() for t in tensors]
+    ptrs = [_ptr(t) for t in tensors]
     base = min(ptrs)
     end = max(p + t.nbytes for p, t in zip(ptrs, tensors))
     return ops.Buf(base, end - base), [p - base for p in ptrs]
 
 
+def _current_device() -> int:
+    import torch
+    return torch.cuda.current_device()
+
+
 class ReedSolomon:
     """ReedSolomon::new(data_shards, parity_shards) of reed-solomon-erasure,
-    with the generator of `layout`, over device shards."""
+    with the generator of `layout`, over device or host shards."""
 
     def __init__(self, data_shards: int, parity_shards: int, layout: str = "vandermonde",
                  decode_cache: int = 64):
@@ -145,9 +169,21 @@ class ReedSolomon:
         for i, s in enumerate(shards):
             if s.nbytes != n:
                 raise ValueError(f"incorrect shard size: shard {i} is {s.nbytes} bytes, expected {n}")
-            if not s.is_contiguous():
+            if not _contiguous(s):
                 raise ValueError(f"shard {i} is not contiguous")
+        _on_host(shards)
         return n
+
+    def _scattered(self, dst_shards: Sequence, n: int, plan: ops.Plan, srcs: Sequence,
+                   stream) -> None:
+        dst, starts = _region(dst_shards)
+        if not _on_host(list(dst_shards) + list(srcs)):
+            ops.dot_product_matrix_scattered_with(Gf8D, dst, n, starts, plan, list(srcs), stream)
+            return
+        if stream is None:
+            stream = ops._current_stream(_current_device())
+        ops.dot_product_matrix_scattered_with_host(Gf8D, dst, n, starts, plan, list(srcs), stream)
+        _check(_lib.lib().fgf_stream_synchronize(int(stream)))
 
     def encode(self, shards: Sequence, stream=None) -> None:
         """Writes the parity shards from the data shards."""
@@ -161,14 +197,23 @@ class ReedSolomon:
             raise ValueError(f"expected {k} data and {m} parity shards, got {len(data)} and {len(parity)}")
         n = self._shard_len(list(data) + list(parity))
         _check_disjoint(parity, data)
-        dst, starts = _region(parity)
-        ops.dot_product_matrix_scattered_with(Gf8D, dst, n, starts, self._encode, list(data), stream)
+        self._scattered(parity, n, self._encode, data, stream)
 
     def verify(self, shards: Sequence, stream=None) -> bool:
         """True when the parity shards hold the parity of the data shards."""
         import torch
         n = self._shard_len(shards)
         k, m = self.data_shards, self.parity_shards
+        if _on_host(shards):
+            import numpy as np
+            fresh = np.empty((m, n), dtype=np.uint8)
+            if stream is None:
+                stream = ops._current_stream(_current_device())
+            ops.dot_product_matrix_with_host(Gf8D, fresh, n, m, self._encode, list(shards[:k]), stream)
+            _check(_lib.lib().fgf_stream_synchronize(int(stream)))
+            as_bytes = lambda t: np.frombuffer(memoryview(t.numpy() if hasattr(t, "numpy") else t)
+                                               .cast("B"), dtype=np.uint8)
+            return all(np.array_equal(fresh[j], as_bytes(shards[k + j])) for j in range(m))
         fresh = torch.empty((m, n), dtype=torch.uint8, device=shards[0].device)
         ops.dot_product_matrix_with(Gf8D, fresh, n, m, self._encode, list(shards[:k]), stream)
         if stream is not None:
@@ -211,9 +256,7 @@ class ReedSolomon:
         survivors = tuple(have[:k])
         _check_disjoint([shards[i] for i in missing], [shards[i] for i in survivors])
         plan = self._decode_plan(survivors, missing)
-        dst, starts = _region([shards[i] for i in missing])
-        ops.dot_product_matrix_scattered_with(Gf8D, dst, n, starts, plan,
-                                              [shards[i] for i in survivors], stream)
+        self._scattered([shards[i] for i in missing], n, plan, [shards[i] for i in survivors], stream)
 
     def reconstruct(self, shards: Sequence, present: Sequence[bool], stream=None) -> None:
         """Rebuilds every shard whose flag is False from the present ones."""

@@ -26,6 +26,7 @@ __all__ = [
     "dot_product_matrix_with", "dot_product_matrix_with_host", "mul_add_matrix_with_host",
     "mul_add_matrix_scattered",
     "mul_add_matrix_scattered_with", "dot_product_matrix_scattered_with",
+    "mul_add_matrix_scattered_with_host", "dot_product_matrix_scattered_with_host",
     "mul_elementwise", "pack", "unpack", "pack_to_vec", "Panic",
 ]
 
@@ -368,6 +369,40 @@ def mul_add_matrix_with_host(field: Field, rows, row_len: int, nrows: int, plan:
     products, chunks of the rows going up beside their sources."""
     _host_matrix(_lib.lib().fgf_mul_add_matrix_with_host, field, rows, row_len, nrows, plan,
                  srcs, stream)
+
+
+def _host_scattered(fn, field: Field, dst, row_len: int, row_starts: Sequence[int], plan: Plan,
+                    srcs: Sequence, stream) -> None:
+    d, dl = _hbuf(dst)
+    starts = (ctypes.c_size_t * max(len(row_starts), 1))(*row_starts)
+    pl = [_hbuf(x) for x in srcs]
+    n = max(len(pl), 1)
+    ptrs = (ctypes.c_void_p * n)(*[q for q, _ in pl])
+    lens = (ctypes.c_size_t * n)(*[ln for _, ln in pl])
+    if stream is None:
+        import torch
+        stream = _current_stream(torch.cuda.current_device())
+    _check(fn(field.id, d, dl, row_len, starts, len(row_starts), plan._h, ptrs, lens, len(srcs),
+              int(stream)))
+
+
+def mul_add_matrix_scattered_with_host(field: Field, dst, row_len: int, row_starts: Sequence[int],
+                                       plan: Plan, srcs: Sequence, stream=None) -> None:
+    """mul_add_matrix_scattered_with over host buffers (no counterpart in the
+    reference): row j at dst[row_starts[j]:][:row_len] ^= sum_t c[t][j] *
+    srcs[t], through the host-buffer pipeline. Synchronize `stream` before
+    reading `dst`."""
+    _host_scattered(_lib.lib().fgf_mul_add_matrix_scattered_with_host, field, dst, row_len,
+                    row_starts, plan, srcs, stream)
+
+
+def dot_product_matrix_scattered_with_host(field: Field, dst, row_len: int,
+                                           row_starts: Sequence[int], plan: Plan, srcs: Sequence,
+                                           stream=None) -> None:
+    """The overwriting scattered form over host buffers: row j becomes
+    sum_t c[t][j] * srcs[t]. Synchronize `stream` before reading `dst`."""
+    _host_scattered(_lib.lib().fgf_dot_product_matrix_scattered_with_host, field, dst, row_len,
+                    row_starts, plan, srcs, stream)
 
 
 def mul_add_matrix_scattered(field: Field, dst, row_len: int, row_starts: Sequence[int], terms,

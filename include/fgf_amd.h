@@ -193,6 +193,21 @@ int fgf_mul_add_matrix_with_host(fgf_field field, uint8_t *rows, size_t rows_len
                                  size_t row_len, size_t nrows, const fgf_plan *plan,
                                  const uint8_t *const *srcs, const size_t *src_lens,
                                  size_t nsrcs, void *stream);
+/* The prepared scattered forms over HOST buffers, through the same
+ * pipeline: row j at dst + row_starts[j] on the host, sources on the host.
+ * No counterpart in the reference (whose scattered op has no prepared form);
+ * they let a host-buffer caller rebuild erased shards in place. Accumulate:
+ * each chunk of the rows goes up beside its sources; overwrite: the rows
+ * become the products (zero with no sources). Checks and messages as the
+ * device scattered forms. The rows are written when `stream` completes. */
+int fgf_mul_add_matrix_scattered_with_host(fgf_field field, uint8_t *dst, size_t dst_len,
+                                           size_t row_len, const size_t *row_starts, size_t nrows,
+                                           const fgf_plan *plan, const uint8_t *const *srcs,
+                                           const size_t *src_lens, size_t nsrcs, void *stream);
+int fgf_dot_product_matrix_scattered_with_host(fgf_field field, uint8_t *dst, size_t dst_len,
+                                               size_t row_len, const size_t *row_starts, size_t nrows,
+                                               const fgf_plan *plan, const uint8_t *const *srcs,
+                                               const size_t *src_lens, size_t nsrcs, void *stream);
 /* Waits for the work queued on `stream` (NULL: the null stream). No
  * counterpart in the reference, whose ops return with their output written:
  * a caller keeping that contract over the host-buffer form calls this after

@@ -275,3 +275,35 @@ def test_compare_bench_matrix_overwrite_equals_fill_and_mul_add(nrows):
         flat = [cols[t][j] for t in range(k) for j in range(nrows)]
         ol.mul_add_matrix(name, want, L, nrows, flat, [host(s) for s in srcs], True)
         assert np.array_equal(host(new), want)
+
+
+@pytest.mark.parametrize("pinned", [False, True], ids=["pageable", "pinned"])
+def test_host_shards_match_device_shards(pinned):
+    """Host shards (numpy arrays, or pinned CPU tensors) through the
+    host-buffer pipeline: the same parity as device shards, written when the
+    call returns; rebuild in place; verify; mixed host and device refused."""
+    k, m, L = 10, 4, (9 << 20) + 4096 + 16
+    rs = interop.ReedSolomon(k, m)
+    data, dshards = shard_set(k, m, L, 31)
+    rs.encode(dshards)
+    want = [host(s) for s in dshards]
+    if pinned:
+        hshards = [torch.from_numpy(d.copy()).pin_memory() for d in data] + \
+                  [torch.full((L,), 0xA5, dtype=torch.uint8).pin_memory() for _ in range(m)]
+        arr = lambda t: t.numpy()
+    else:
+        hshards = [d.copy() for d in data] + [np.full(L, 0xA5, np.uint8) for _ in range(m)]
+        arr = lambda t: t
+    rs.encode(hshards)                                   # written on return: no synchronize
+    for i in range(k + m):
+        assert np.array_equal(arr(hshards[i]), want[i]), i
+    assert rs.verify(hshards)
+    lost = [2, 9, 11, 13]
+    for i in lost:
+        arr(hshards[i])[:] = 0
+    assert not rs.verify(hshards)
+    rs.reconstruct(hshards, [i not in lost for i in range(k + m)])
+    for i in range(k + m):
+        assert np.array_equal(arr(hshards[i]), want[i]), i
+    with pytest.raises(ValueError, match="all be device tensors or all host"):
+        rs.encode(dshards[:k] + list(hshards[k:]))

@@ -309,6 +309,40 @@ def test_host_buffer_accumulate_matches_device_form(f):
         assert np.array_equal(got, want), (row_len, pinned)
 
 
+@pytest.mark.parametrize("f", [Gf8D, Gf32], ids=lambda f: NAMES[f.id])
+def test_host_buffer_scattered_forms_match_device_forms(f):
+    # Rows out of order with gaps, in a host buffer: the accumulating and
+    # overwriting host forms equal the device forms on a copy, over several
+    # pipeline chunks (pinned) and one chunk (pageable); gaps keep their bytes.
+    k = 9
+    for row_len, pinned in (((9 << 20) + 4096 + 64, True), (4096 + 32, False)):
+        starts = [2 * row_len + 64, 0, row_len + 32]
+        m, size = len(starts), 3 * row_len + 128
+        srcs = [noise(row_len, 0x1400 + t) for t in range(k)]
+        init = noise(size, 0x1500 + f.id)
+        plan = ops.Plan.matrix(f, k, m, rand_coeffs(f, k * m, 0x1600 + f.id))
+        for accumulate in (True, False):
+            want = dev(init)
+            dev_fn = ops.mul_add_matrix_scattered_with if accumulate else ops.dot_product_matrix_scattered_with
+            dev_fn(f, want, row_len, starts, plan, [dev(x) for x in srcs])
+            want = host(want)
+            if pinned:
+                h_srcs = [torch.from_numpy(x).pin_memory() for x in srcs]
+                h_dst = torch.from_numpy(init.copy()).pin_memory()
+            else:
+                h_srcs, h_dst = srcs, init.copy()
+            host_fn = (ops.mul_add_matrix_scattered_with_host if accumulate
+                       else ops.dot_product_matrix_scattered_with_host)
+            host_fn(f, h_dst, row_len, starts, plan, h_srcs)
+            torch.cuda.synchronize()
+            got = h_dst.numpy() if pinned else h_dst
+            assert np.array_equal(got, want), (row_len, pinned, accumulate)
+            assert np.array_equal(got[row_len:row_len + 32], init[row_len:row_len + 32])
+    with pytest.raises(ops.Panic, match="overlap"):
+        ops.dot_product_matrix_scattered_with_host(f, np.zeros(64, np.uint8), 16, [0, 8],
+                                                   ops.Plan.matrix(f, 0, 2, []), [])
+
+
 @pytest.mark.parametrize("f", ALL_FIELDS, ids=lambda f: NAMES[f.id])
 def test_matrix_scattered_matches_contiguous(f):
     # tests/ops.rs:406-478: rows separated by element-aligned gaps of 7
