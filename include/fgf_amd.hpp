@@ -430,6 +430,32 @@ void dot_product_matrix_scattered_with(Bytes dst, std::size_t row_len,
                                                         srcs.size(), stream));
 }
 
+/* The two prepared scattered forms over HOST buffers (the host-buffer
+ * pipeline); the rows are written when `stream` completes. */
+template <class F>
+void mul_add_matrix_scattered_with_host(Bytes dst, std::size_t row_len,
+                                        const std::vector<std::size_t> &row_starts,
+                                        const Plan<F> &plan, const std::vector<CBytes> &srcs,
+                                        void *stream = nullptr) {
+    detail::Srcs s(srcs);
+    detail::check(fgf_mul_add_matrix_scattered_with_host(F::ID, dst.ptr, dst.len, row_len,
+                                                         row_starts.data(), row_starts.size(),
+                                                         plan.raw(), s.ptrs.data(), s.lens.data(),
+                                                         srcs.size(), stream));
+}
+
+template <class F>
+void dot_product_matrix_scattered_with_host(Bytes dst, std::size_t row_len,
+                                            const std::vector<std::size_t> &row_starts,
+                                            const Plan<F> &plan, const std::vector<CBytes> &srcs,
+                                            void *stream = nullptr) {
+    detail::Srcs s(srcs);
+    detail::check(fgf_dot_product_matrix_scattered_with_host(F::ID, dst.ptr, dst.len, row_len,
+                                                             row_starts.data(), row_starts.size(),
+                                                             plan.raw(), s.ptrs.data(),
+                                                             s.lens.data(), srcs.size(), stream));
+}
+
 /* mul_elementwise (src/ops.rs:953-957): dst[i] = a[i] * b[i]. */
 template <class F>
 void mul_elementwise(Bytes dst, CBytes a, CBytes b, void *stream = nullptr) {

@@ -386,6 +386,43 @@ void check_dot_product_matrix_scattered(uint64_t seed) {
                                     sp.data(), k);
     REQUIRE(d.get() == want, "overwriting scattered");
 }
+// The same through the host-buffer forms, accumulating then overwriting.
+template <class F>
+void check_scattered_host(uint64_t seed) {
+    const size_t row_len = 4100 * F::BYTES, nrows = 3, k = 4;
+    const std::vector<size_t> starts = {2 * row_len + 4 * F::BYTES, 0, row_len + 2 * F::BYTES};
+    std::vector<Vec> srcs;
+    std::vector<CBytes> views;
+    std::vector<typename F::Elem> flat;
+    for (size_t t = 0; t < k; ++t) srcs.push_back(noise(row_len, seed + t));
+    for (size_t t = 0; t < k; ++t) {
+        views.push_back({srcs[t].data(), srcs[t].size()});
+        auto col = rand_elems<F>(nrows, seed + 10 + t);
+        flat.insert(flat.end(), col.begin(), col.end());
+    }
+    const auto plan = Plan<F>::matrix(k, nrows, flat);
+    std::vector<uint64_t> c64(flat.begin(), flat.end());
+    std::vector<const uint8_t *> sp;
+    for (const Vec &s : srcs) sp.push_back(s.data());
+    const Vec init = noise(3 * row_len + 8 * F::BYTES, seed + 20);
+    Vec got = init, want = init;
+    ops::mul_add_matrix_scattered_with_host<F>({got.data(), got.size()}, row_len, starts, plan, views);
+    detail::check(fgf_stream_synchronize(nullptr));
+    oracle_mul_add_matrix_scattered(F::ID, want.data(), row_len, starts.data(), nrows, c64.data(),
+                                    sp.data(), k);
+    REQUIRE(got == want, "host scattered accumulate");
+    got = init;
+    want = init;
+    ops::dot_product_matrix_scattered_with_host<F>({got.data(), got.size()}, row_len, starts, plan, views);
+    detail::check(fgf_stream_synchronize(nullptr));
+    for (size_t s : starts) std::fill(want.begin() + s, want.begin() + s + row_len, 0);
+    oracle_mul_add_matrix_scattered(F::ID, want.data(), row_len, starts.data(), nrows, c64.data(),
+                                    sp.data(), k);
+    REQUIRE(got == want, "host scattered overwrite");
+}
+TEST(gf8d_scattered_host_forms) { check_scattered_host<Gf8D>(0x85); }
+TEST(gf16_scattered_host_forms) { check_scattered_host<Gf16>(0x86); }
+
 TEST(gf8d_dot_product_matrix_scattered_overwrites) { check_dot_product_matrix_scattered<Gf8D>(0x83); }
 TEST(gf32_dot_product_matrix_scattered_overwrites) { check_dot_product_matrix_scattered<Gf32>(0x84); }
 
