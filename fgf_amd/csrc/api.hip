@@ -205,6 +205,32 @@ bool plan_ok(int f, const fgf_plan *p, const char *name, int *rc) {
 
 hipStream_t S(void *s) { return static_cast<hipStream_t>(s); }
 
+// The reference takes destinations as `&mut [u8]` and sources as `&[u8]`, so
+// the borrow rules keep them apart. Over raw pointers a source overlapping a
+// destination would race with the kernel's stores; it is refused. Sources
+// may alias each other.
+bool spans_overlap(const void *a, size_t alen, const void *b, size_t blen) {
+    const uintptr_t x = (uintptr_t)a, y = (uintptr_t)b;
+    return alen != 0 && blen != 0 && x < y + blen && y < x + alen;
+}
+
+bool apart(const char *name, const uint8_t *dst, size_t dst_len, const uint8_t *const *srcs,
+           const size_t *lens, size_t n, int *rc) {
+    for (size_t i = 0; i < n; ++i)
+        if (spans_overlap(dst, dst_len, srcs[i], lens[i])) {
+            *rc = fail("%s: source %zu overlaps the destination", name, i);
+            return false;
+        }
+    return true;
+}
+
+bool rows_apart(const char *name, const uint8_t *dst, const size_t *row_starts, size_t nrows,
+                size_t row_len, const uint8_t *const *srcs, const size_t *lens, size_t n, int *rc) {
+    for (size_t j = 0; j < nrows; ++j)
+        if (!apart(name, dst + row_starts[j], row_len, srcs, lens, n, rc)) return false;
+    return true;
+}
+
 #define FIELD_GUARD(f)                                                  \
     do {                                                                \
         if (!field_ok(f)) return fail("unknown field id %d", (int)(f)); \
@@ -461,6 +487,7 @@ int fgf_add_assign(fgf_field f, uint8_t *dst, size_t dst_len, const uint8_t *src
     FIELD_GUARD(f);
     int rc = 0;
     if (!check_pair(f, "add_assign", "dst", dst_len, "src", src_len, &rc)) return rc;
+    if (!apart("add_assign", dst, dst_len, &src, &src_len, 1, &rc)) return rc;
     return single(f, dst, src, dst_len, 1, false, S(stream));
 }
 
@@ -470,6 +497,7 @@ int fgf_mul_add(fgf_field f, uint8_t *dst, size_t dst_len, uint64_t coeff, const
     int rc = 0;
     if (!check_pair(f, "mul_add", "dst", dst_len, "src", src_len, &rc)) return rc;
     if (!coeff_in_field(f, coeff, "mul_add", &rc)) return rc;
+    if (!apart("mul_add", dst, dst_len, &src, &src_len, 1, &rc)) return rc;
     if (coeff == 0) return FGF_OK;
     return single(f, dst, src, dst_len, coeff, false, S(stream));
 }
@@ -481,6 +509,7 @@ int fgf_mul_add_with(fgf_field f, uint8_t *dst, size_t dst_len, const fgf_plan *
     if (!check_pair(f, "mul_add_with", "dst", dst_len, "src", src_len, &rc)) return rc;
     if (!plan_ok(f, plan, "mul_add_with", &rc)) return rc;
     if (index >= plan->values.size()) return fail("mul_add_with: coefficient index %zu out of bounds", index);
+    if (!apart("mul_add_with", dst, dst_len, &src, &src_len, 1, &rc)) return rc;
     if (plan->values[index] == 0) return FGF_OK;
     return single_plan(plan, index, dst, src, dst_len, false, S(stream));
 }
@@ -491,6 +520,7 @@ int fgf_mul_into(fgf_field f, uint8_t *dst, size_t dst_len, uint64_t coeff, cons
     int rc = 0;
     if (!check_pair(f, "mul_into", "dst", dst_len, "src", src_len, &rc)) return rc;
     if (!coeff_in_field(f, coeff, "mul_into", &rc)) return rc;
+    if (!apart("mul_into", dst, dst_len, &src, &src_len, 1, &rc)) return rc;
     if (coeff == 0) return fill_zero(dst, dst_len, S(stream));
     return single(f, dst, src, dst_len, coeff, true, S(stream));
 }
@@ -502,6 +532,7 @@ int fgf_mul_into_with(fgf_field f, uint8_t *dst, size_t dst_len, const fgf_plan 
     if (!check_pair(f, "mul_into_with", "dst", dst_len, "src", src_len, &rc)) return rc;
     if (!plan_ok(f, plan, "mul_into_with", &rc)) return rc;
     if (index >= plan->values.size()) return fail("mul_into_with: coefficient index %zu out of bounds", index);
+    if (!apart("mul_into_with", dst, dst_len, &src, &src_len, 1, &rc)) return rc;
     if (plan->values[index] == 0) return fill_zero(dst, dst_len, S(stream));
     return single_plan(plan, index, dst, src, dst_len, true, S(stream));
 }
@@ -544,6 +575,7 @@ int fgf_mul_add_scatter(fgf_field f, uint8_t *rows, size_t rows_len, size_t row_
     if (rows_len < used)
         return fail("mul_add_scatter: rows is %zu bytes but %zu rows of %zu bytes need %zu", rows_len,
                     ncoeffs, row_len, used);
+    if (!apart("mul_add_scatter", rows, used, &src, &src_len, 1, &rc)) return rc;
     std::vector<uint64_t> c;
     if (!decode_coeffs(f, coeffs, ncoeffs, c, &rc)) return rc;
     if (all_zero(c) || row_len == 0) return FGF_OK;
@@ -566,6 +598,7 @@ int fgf_mul_add_scatter_with(fgf_field f, uint8_t *rows, size_t rows_len, size_t
     if (rows_len < used)
         return fail("mul_add_scatter_with: rows is %zu bytes but %zu rows of %zu bytes need %zu",
                     rows_len, n, row_len, used);
+    if (!apart("mul_add_scatter_with", rows, used, &src, &src_len, 1, &rc)) return rc;
     if (all_zero(plan->values) || row_len == 0) return FGF_OK;
     auto r = contiguous_rows(rows, n, row_len);
     return run_matrix(f, plan->d_tab, 0, entry_dw(plan->nb), r.data(), n, &src, 1, row_len, false,
@@ -601,6 +634,7 @@ int fgf_mul_add_gather(fgf_field f, uint8_t *dst, size_t dst_len, const void *co
     FIELD_GUARD(f);
     int rc = 0;
     if (!check_gather(f, "mul_add_gather", dst_len, ncoeffs, src_lens, nsrcs, false, false, &rc)) return rc;
+    if (!apart("mul_add_gather", dst, dst_len, srcs, src_lens, nsrcs, &rc)) return rc;
     std::vector<uint64_t> c;
     if (!decode_coeffs(f, coeffs, ncoeffs, c, &rc)) return rc;
     if (nsrcs == 0 || all_zero(c) || dst_len == 0) return FGF_OK;
@@ -620,6 +654,7 @@ int fgf_mul_add_gather_with(fgf_field f, uint8_t *dst, size_t dst_len, const fgf
     if (!check_gather(f, "mul_add_gather_with", dst_len, plan->values.size(), src_lens, nsrcs, true,
                       true, &rc))
         return rc;
+    if (!apart("mul_add_gather_with", dst, dst_len, srcs, src_lens, nsrcs, &rc)) return rc;
     if (nsrcs == 0 || all_zero(plan->values) || dst_len == 0) return FGF_OK;
     return run_matrix(f, plan->d_tab, entry_dw(plan->nb), 0, &dst, 1, srcs, nsrcs, dst_len, false,
                       S(stream));
@@ -630,6 +665,7 @@ int fgf_dot_product(fgf_field f, uint8_t *dst, size_t dst_len, const void *coeff
     FIELD_GUARD(f);
     int rc = 0;
     if (!check_gather(f, "dot_product", dst_len, ncoeffs, src_lens, nsrcs, true, false, &rc)) return rc;
+    if (!apart("dot_product", dst, dst_len, srcs, src_lens, nsrcs, &rc)) return rc;
     std::vector<uint64_t> c;
     if (!decode_coeffs(f, coeffs, ncoeffs, c, &rc)) return rc;
     if (nsrcs == 0 || all_zero(c)) return fill_zero(dst, dst_len, S(stream));
@@ -651,6 +687,7 @@ int fgf_dot_product_with(fgf_field f, uint8_t *dst, size_t dst_len, const fgf_pl
     if (!check_gather(f, "dot_product_with", dst_len, plan->values.size(), src_lens, nsrcs, true, true,
                       &rc))
         return rc;
+    if (!apart("dot_product_with", dst, dst_len, srcs, src_lens, nsrcs, &rc)) return rc;
     if (nsrcs == 0 || all_zero(plan->values)) return fill_zero(dst, dst_len, S(stream));
     if (plan->values.size() == 1)
         return fgf_mul_into_with(f, dst, dst_len, plan, 0, srcs[0], src_lens[0], stream);
@@ -673,6 +710,7 @@ int fgf_mul_add_matrix(fgf_field f, uint8_t *rows, size_t rows_len, size_t row_l
     if (!check_matrix_terms(f, "mul_add_matrix", rows_len, row_len, nrows, term_coeffs, term_ncoeffs,
                             term_src_lens, nterms, &used, &rc))
         return rc;
+    if (!apart("mul_add_matrix", rows, used, term_srcs, term_src_lens, nterms, &rc)) return rc;
     if (nrows == 0 || nterms == 0 || row_len == 0) return FGF_OK;
     std::vector<uint64_t> flat;
     if (!flatten_terms(f, term_coeffs, nterms, nrows, flat, &rc)) return rc;
@@ -693,6 +731,7 @@ int fgf_mul_add_matrix_with(fgf_field f, uint8_t *rows, size_t rows_len, size_t 
     if (!check_matrix_plan(f, "mul_add_matrix_with", rows_len, row_len, nrows, plan, src_lens, nsrcs,
                            &used, &rc))
         return rc;
+    if (!apart("mul_add_matrix_with", rows, used, srcs, src_lens, nsrcs, &rc)) return rc;
     if (nrows == 0 || nsrcs == 0 || row_len == 0) return FGF_OK;
     const size_t e = entry_dw(plan->nb);
     auto r = contiguous_rows(rows, nrows, row_len);
@@ -710,6 +749,7 @@ int fgf_dot_product_matrix(fgf_field f, uint8_t *rows, size_t rows_len, size_t r
     if (!check_matrix_terms(f, "dot_product_matrix", rows_len, row_len, nrows, term_coeffs,
                             term_ncoeffs, term_src_lens, nterms, &used, &rc))
         return rc;
+    if (!apart("dot_product_matrix", rows, used, term_srcs, term_src_lens, nterms, &rc)) return rc;
     if (nrows == 0) return FGF_OK;
     if (nterms == 0) return fill_zero(rows, used, S(stream));
     if (row_len == 0) return FGF_OK;
@@ -732,6 +772,7 @@ int fgf_dot_product_matrix_with(fgf_field f, uint8_t *rows, size_t rows_len, siz
     if (!check_matrix_plan(f, "dot_product_matrix_with", rows_len, row_len, nrows, plan, src_lens,
                            nsrcs, &used, &rc))
         return rc;
+    if (!apart("dot_product_matrix_with", rows, used, srcs, src_lens, nsrcs, &rc)) return rc;
     if (nrows == 0) return FGF_OK;
     if (nsrcs == 0) return fill_zero(rows, used, S(stream));
     if (row_len == 0) return FGF_OK;
@@ -884,6 +925,7 @@ int host_matrix(int f, const char *name, uint8_t *rows, size_t rows_len, size_t 
     size_t used;
     if (!check_matrix_plan(f, name, rows_len, row_len, nrows, plan, src_lens, nsrcs, &used, &rc))
         return rc;
+    if (!apart(name, rows, used, srcs, src_lens, nsrcs, &rc)) return rc;
     return host_pipeline(f, name, contiguous_rows(rows, nrows, row_len), row_len, plan, srcs, nsrcs,
                          overwrite, s);
 }
@@ -903,6 +945,7 @@ int host_scattered(int f, const char *name, uint8_t *dst, size_t dst_len, size_t
     for (size_t i = 0; i < nsrcs; ++i)
         if (src_lens[i] != row_len)
             return fail("%s: source %zu is %zu bytes, expected %zu", name, i, src_lens[i], row_len);
+    if (!rows_apart(name, dst, row_starts, nrows, row_len, srcs, src_lens, nsrcs, &rc)) return rc;
     std::vector<uint8_t *> r(nrows);
     for (size_t j = 0; j < nrows; ++j) r[j] = dst + row_starts[j];
     return host_pipeline(f, name, r, row_len, plan, srcs, nsrcs, overwrite, s);
@@ -1003,6 +1046,8 @@ int fgf_mul_add_matrix_scattered(fgf_field f, uint8_t *dst, size_t dst_len, size
         if (term_src_lens[t] != row_len)
             return fail("%s: source is %zu bytes, expected %zu", name, term_src_lens[t], row_len);
     }
+    if (!rows_apart(name, dst, row_starts, nrows, row_len, term_srcs, term_src_lens, nterms, &rc))
+        return rc;
     if (nrows == 0 || nterms == 0 || row_len == 0) return FGF_OK;
     std::vector<uint64_t> flat;
     if (!flatten_terms(f, term_coeffs, nterms, nrows, flat, &rc)) return rc;
@@ -1032,6 +1077,7 @@ int scattered_with(int f, const char *name, uint8_t *dst, size_t dst_len, size_t
     for (size_t i = 0; i < nsrcs; ++i)
         if (src_lens[i] != row_len)
             return fail("%s: source %zu is %zu bytes, expected %zu", name, i, src_lens[i], row_len);
+    if (!rows_apart(name, dst, row_starts, nrows, row_len, srcs, src_lens, nsrcs, &rc)) return rc;
     if (nrows == 0 || row_len == 0) return FGF_OK;
     if (nsrcs == 0) {
         if (!overwrite) return FGF_OK;
@@ -1071,6 +1117,9 @@ int fgf_mul_elementwise(fgf_field f, uint8_t *dst, size_t dst_len, const uint8_t
     int rc = 0;
     if (!check_pair(f, "mul_elementwise", "dst", dst_len, "a", a_len, &rc)) return rc;
     if (!check_pair(f, "mul_elementwise", "dst", dst_len, "b", b_len, &rc)) return rc;
+    if (!apart("mul_elementwise", dst, dst_len, &a, &a_len, 1, &rc) ||
+        !apart("mul_elementwise", dst, dst_len, &b, &b_len, 1, &rc))
+        return rc;
     hipError_t e = launch_elementwise(f, dst, a, b, dst_len, S(stream));
     return e == hipSuccess ? FGF_OK : device_fail("elementwise launch", e);
 }

@@ -22,8 +22,9 @@
  *    the call returns FGF_EINVAL, touches nothing, and fgf_last_error()
  *    holds the reference's panic message (thread-local).
  *  - Destination and source buffers must not overlap (the reference's
- *    borrow rules); sources may alias each other. `fgf_mul_assign` is in
- *    place by definition.
+ *    borrow rules): a source overlapping a destination row is refused with
+ *    FGF_EINVAL ("<op>: source i overlaps the destination"). Sources may
+ *    alias each other. `fgf_mul_assign` is in place by definition.
  *  - The `_with` forms allocate nothing per call. Plans hold their tables in
  *    device memory on the device that was current when they were built.
  */

@@ -306,3 +306,19 @@ def test_server_config_is_validated_on_the_host():
         Server(slots=0)
     with pytest.raises(Panic, match="dwords must be 4 or 8"):
         Server(dwords=3)
+
+
+def test_sources_overlapping_a_destination_are_refused():
+    # The reference's borrow rules keep destinations and sources apart; over
+    # raw pointers the ABI refuses the overlap before touching anything.
+    buf = z(96)
+    with pytest.raises(Panic, match="mul_add: source 0 overlaps the destination"):
+        ops.mul_add(Gf8B, buf[:32], 3, buf[16:48])
+    with pytest.raises(Panic, match="dot_product: source 1 overlaps the destination"):
+        ops.dot_product(Gf8B, buf[:32], [2, 3], [buf[64:96], buf[31:63]])
+    with pytest.raises(Panic, match="dot_product_matrix: source 0 overlaps the destination"):
+        ops.dot_product_matrix(Gf8B, buf[:64], 32, 2, [([1, 2], buf[40:72])])
+    with pytest.raises(Panic, match="mul_add_matrix_scattered: source 0 overlaps"):
+        ops.mul_add_matrix_scattered(Gf8B, buf, 16, [64, 0], [([1, 2], buf[70:86])])
+    with pytest.raises(Panic, match="mul_elementwise: source 0 overlaps"):
+        ops.mul_elementwise(Gf8B, buf[:32], buf[:32], buf[32:64])

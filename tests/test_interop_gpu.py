@@ -307,3 +307,25 @@ def test_host_shards_match_device_shards(pinned):
         assert np.array_equal(arr(hshards[i]), want[i]), i
     with pytest.raises(ValueError, match="all be device tensors or all host"):
         rs.encode(dshards[:k] + list(hshards[k:]))
+
+
+def test_sources_may_alias_each_other_and_sit_between_rows():
+    """Only a source overlapping a written row is refused: the same source
+    twice, and sources inside the gaps of a scattered destination, work."""
+    L = 4096
+    region = dev(ol.noise(4 * L, 51))
+    src = region[L:2 * L]
+    plan = ops.Plan.matrix(Gf8D, 2, 2, [3, 5, 7, 11])
+    before = host(region).copy()
+    ops.dot_product_matrix_scattered_with(Gf8D, region, L, [0, 2 * L], plan, [src, src])
+    want = before.copy()
+    s = before[L:2 * L]
+    for j, start in enumerate((0, 2 * L)):
+        row = np.zeros(L, np.uint8)
+        ol.mul_add("gf8d", row, [3, 5][j], s)        # plan (t = 0, j)
+        ol.mul_add("gf8d", row, [7, 11][j], s)       # plan (t = 1, j)
+        want[start:start + L] = row
+    assert np.array_equal(host(region), want)
+    with pytest.raises(ops.Panic, match="overlaps the destination"):
+        ops.dot_product_matrix_scattered_with(Gf8D, region, L, [0, 2 * L], plan,
+                                              [src, region[2 * L + 8:3 * L + 8]])
