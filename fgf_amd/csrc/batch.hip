@@ -309,6 +309,13 @@ int fgf_batch_create(fgf_field f, const fgf_stripe *stripes, size_t n, fgf_batch
                 if ((x < y ? y - x : x - y) < st.row_len)
                     return bfail("batch stripe %zu: rows %zu and %zu overlap for %zu-byte rows", s, a, c, st.row_len);
             }
+        // No source may overlap a row (the kernel's stores would race with it).
+        for (size_t t = 0; t < p->rows; ++t)
+            for (size_t j = 0; j < p->cols; ++j) {
+                const uintptr_t x = (uintptr_t)st.srcs[t], y = (uintptr_t)st.rows[j];
+                if (st.row_len && x < y + st.row_len && y < x + st.row_len)
+                    return bfail("batch stripe %zu: source %zu overlaps row %zu", s, t, j);
+            }
         nptrs += p->rows + p->cols;
     }
     auto *b = new (std::nothrow) fgf_batch;

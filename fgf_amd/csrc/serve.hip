@@ -541,6 +541,14 @@ int make_req(fgf_server *s, const fgf_request *r, Req &q) {
         return sfail("fgf_server_submit: row_len %zu is not a nonzero multiple of 16", r->row_len);
     for (uint32_t i = 0; i < r->k + r->m; ++i)
         if ((uintptr_t)r->shards[i] & 15u) return sfail("fgf_server_submit: shard %u is not 16-byte aligned", i);
+    // The shards of a stripe are distinct buffers: an overlap would race the
+    // kernel's stores with its loads.
+    std::vector<uintptr_t> spans(r->k + r->m);
+    for (uint32_t i = 0; i < r->k + r->m; ++i) spans[i] = (uintptr_t)r->shards[i];
+    std::sort(spans.begin(), spans.end());
+    for (size_t i = 1; i < spans.size(); ++i)
+        if (spans[i] - spans[i - 1] < r->row_len)
+            return sfail("fgf_server_submit: shards overlap for %zu-byte rows", r->row_len);
     q.r = *r;
     q.shards.assign(r->shards, r->shards + r->k + r->m);
     if (r->reconstruct) {

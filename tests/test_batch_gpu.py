@@ -180,3 +180,5 @@ def test_batch_rejects_misaligned_and_overlapping_rows():
         Batch(Gf8D, [Stripe(plan, [buf[1:65]], [buf[1024:1088], buf[2048:2112]], 64)])
     with pytest.raises(Panic, match="overlap"):
         Batch(Gf8D, [Stripe(plan, [buf[:64]], [buf[1024:1088], buf[1056:1120]], 64)])
+    with pytest.raises(Panic, match="source 0 overlaps row 1"):
+        Batch(Gf8D, [Stripe(plan, [buf[2048:2112]], [buf[1024:1088], buf[2080:2144]], 64)])

@@ -98,3 +98,14 @@ def test_request_larger_than_budget_fails_cleanly():
     lat, status = srv.result(t)
     assert status == -3
     srv.close()
+
+
+def test_overlapping_shards_are_refused_at_submit():
+    from fgf_amd.field import Panic
+    buf = torch.zeros(1 << 16, dtype=torch.uint8, device="cuda")
+    L = 4096
+    shards = [buf[i * L:(i + 1) * L] for i in range(5)] + [buf[4 * L + 16:5 * L + 16]]
+    srv = Server(slots=1)
+    with pytest.raises(Panic, match="shards overlap"):
+        srv.submit(Request(Gf8D, 4, 2, shards, L))
+    srv.close()
