@@ -78,11 +78,20 @@ def _coeff_bytes(field: Field, coeffs: Sequence[int]) -> ctypes.Array:
 
 
 def _ptr_array(bufs: Sequence) -> tuple[ctypes.Array, ctypes.Array]:
-    n = max(len(bufs), 1)
-    pl = [_buf(b) for b in bufs]
-    ptrs = (ctypes.c_void_p * n)(*[p for p, _ in pl])
-    lens = (ctypes.c_size_t * n)(*[ln for _, ln in pl])
-    return ptrs, lens
+    # One pass, no intermediate tuples: this runs on every call of the
+    # multi-source ops, where it is most of the Python mirror's own cost.
+    ptrs, lens = [], []
+    for b in bufs:
+        if b.__class__ is Buf:
+            ptrs.append(b.ptr)
+            lens.append(b.nbytes)
+            continue
+        if not b.is_contiguous():
+            raise Panic("buffer must be a contiguous byte range")
+        ptrs.append(b.data_ptr())
+        lens.append(b.nbytes)
+    n = max(len(ptrs), 1)
+    return (ctypes.c_void_p * n)(*ptrs), (ctypes.c_size_t * n)(*lens)
 
 
 # ---------------------------------------------------------------------------
