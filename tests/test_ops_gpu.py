@@ -341,6 +341,20 @@ def test_host_buffer_scattered_forms_match_device_forms(f):
     with pytest.raises(ops.Panic, match="overlap"):
         ops.dot_product_matrix_scattered_with_host(f, np.zeros(64, np.uint8), 16, [0, 8],
                                                    ops.Plan.matrix(f, 0, 2, []), [])
+    # no sources: the overwriting form zeroes its rows and leaves the gaps,
+    # the accumulating form changes nothing; no rows: nothing at all
+    buf = np.full(96, 0x5A, np.uint8)
+    ops.mul_add_matrix_scattered_with_host(f, buf, 16, [64, 8], ops.Plan.matrix(f, 0, 2, []), [])
+    torch.cuda.synchronize()
+    assert (buf == 0x5A).all()
+    ops.dot_product_matrix_scattered_with_host(f, buf, 16, [64, 8], ops.Plan.matrix(f, 0, 2, []), [])
+    torch.cuda.synchronize()
+    assert not buf[64:80].any() and not buf[8:24].any()
+    assert (buf[:8] == 0x5A).all() and (buf[24:64] == 0x5A).all() and (buf[80:] == 0x5A).all()
+    ops.dot_product_matrix_scattered_with_host(f, buf, 16, [], ops.Plan.matrix(f, 1, 0, []),
+                                               [np.ones(16, np.uint8)])
+    torch.cuda.synchronize()
+    assert not buf[64:80].any() and (buf[80:] == 0x5A).all()
 
 
 @pytest.mark.parametrize("f", ALL_FIELDS, ids=lambda f: NAMES[f.id])
