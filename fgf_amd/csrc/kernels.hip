@@ -295,6 +295,10 @@ hipError_t launch_segment(const MatrixJob &job, MatrixArgs a, size_t k0, size_t 
 
 }  // namespace
 
+// Columns per launch: a multiple of every block's column span and element
+// size; 8 GiB keeps the generic kernel at 2^31 work-items.
+constexpr uint64_t MAX_SEGMENT = 8ull << 30;
+
 hipError_t launch_matrix(const MatrixJob &job) {
     if (job.m == 0 || job.k == 0 || job.row_len == 0) return hipSuccess;
     const int nb = job.nb;
@@ -317,10 +321,20 @@ hipError_t launch_matrix(const MatrixJob &job) {
         const bool ow = job.overwrite && k0 == 0;
         for (size_t j0 = 0; j0 < job.m; j0 += MMAX) {
             const size_t mn = std::min<size_t>(MMAX, job.m - j0);
+            // Each launch covers at most MAX_SEGMENT columns, so its work-item
+            // count (one per 4 elements in the generic kernel) stays below 2^32.
+            const auto segments = [&](uint64_t c0, uint64_t n, bool vector) {
+                for (uint64_t c = 0; c < n; c += MAX_SEGMENT) {
+                    const hipError_t e = launch_segment(job, a, k0, kn, j0, mn, c0 + c,
+                                                        std::min<uint64_t>(MAX_SEGMENT, n - c), vector, ow);
+                    if (e != hipSuccess) return e;
+                }
+                return hipSuccess;
+            };
             hipError_t e;
-            if ((e = launch_segment(job, a, k0, kn, j0, mn, 0, head, false, ow)) != hipSuccess) return e;
-            if ((e = launch_segment(job, a, k0, kn, j0, mn, head, body, true, ow)) != hipSuccess) return e;
-            if ((e = launch_segment(job, a, k0, kn, j0, mn, tail0, tail, false, ow)) != hipSuccess) return e;
+            if ((e = segments(0, head, false)) != hipSuccess) return e;
+            if ((e = segments(head, body, true)) != hipSuccess) return e;
+            if ((e = segments(tail0, tail, false)) != hipSuccess) return e;
         }
     }
     return hipSuccess;

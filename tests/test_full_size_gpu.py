@@ -127,3 +127,30 @@ def test_gf64_elementwise_2pow26_pairs_properties():
         want = ol.mul_elementwise("gf64", a[off:off + 65536].cpu().numpy(),
                                   b[off:off + 65536].cpu().numpy())
         assert np.array_equal(ab[off:off + 65536].cpu().numpy(), want), off
+
+
+def test_a_row_longer_than_one_launch():
+    """A 17 GiB row runs as several launches of at most 8 GiB of columns
+    (the generic kernel's work-item count must stay below 2^32): aligned
+    buffers on the vector kernel, and a source one byte off its destination's
+    alignment on the generic kernel. Slices at every launch boundary are
+    checked against the oracle."""
+    def host(t):
+        torch.cuda.synchronize()
+        return t.cpu().numpy()
+
+    n = (17 << 30) + 48
+    src = torch.randint(0, 256, (n + 1,), dtype=torch.uint8, device="cuda")
+    dst = torch.randint(0, 256, (n,), dtype=torch.uint8, device="cuda")
+    probes = [0, (8 << 30) - 2048, (16 << 30) - 2048, n - 4096]
+    for off in (0, 1):
+        print(f"\n17 GiB mul_add, source offset {off}", flush=True)
+        s = src[off:off + n]
+        before = [host(dst[p:p + 4096]).copy() for p in probes]
+        srcs = [host(s[p:p + 4096]).copy() for p in probes]
+        ops.mul_add(Gf8D, dst, 0x53, s)
+        for p, b, x in zip(probes, before, srcs):
+            ol.mul_add("gf8d", b, 0x53, x)
+            assert np.array_equal(host(dst[p:p + 4096]), b), (off, p)
+    del src, dst
+    torch.cuda.empty_cache()
