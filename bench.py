@@ -586,6 +586,9 @@ def main() -> int:
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the PCIe-inclusive run and the other workload shapes")
+    ap.add_argument("--preroll-ms", type=float, default=300.0,
+                    help="untimed back-to-back launches for at least this long before the "
+                         "counted warm-up (sustained clocks for the timed launches)")
     ap.add_argument("--row-len", type=int, default=L_HEADLINE)
     ap.add_argument("--calib-copy", action="store_true",
                     help="after timing, run one 1 GiB device copy (a plain copy kernel of known "
@@ -629,10 +632,41 @@ def main() -> int:
                                          [src[t, -probe:].cpu().numpy() for t in range(K)], M)
     assert np.array_equal(rows[:, -probe:].cpu().numpy(), want), "encode mismatch"
 
-    # Warm-up launches run back to back straight into the timed region, so the
-    # timed launches see the clocks of sustained load rather than the power
-    # controller's response to load starting (a dip of some 10 ms that begins
-    # a few ms after an idle GPU starts this kernel).
+    def launch_intervals(n: int) -> list[float]:
+        """Per-launch durations (ms) of n back-to-back launches, from HIP events
+        recorded on the launch stream between them."""
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(n + 1)]
+        ev[0].record(stream)
+        for i in range(n):
+            ops.dot_product_matrix_with(Gf8D, rows.view(-1), L, M, plan, srcs)
+            ev[i + 1].record(stream)
+        torch.cuda.synchronize()
+        return [ev[i].elapsed_time(ev[i + 1]) for i in range(n)]
+
+    # Cold start: the first launches after the GPU has idled through the host
+    # check above, for the record. An idle GPU takes some tens of ms of
+    # sustained load before its clocks settle (a dip of some 10 ms begins a
+    # few ms after load starts), so these are slower than the steady state.
+    time.sleep(0.2)
+    cold = sorted(launch_intervals(max(args.steps, 5)))
+
+    # Pre-roll: back-to-back launches for at least --preroll-ms of wall clock,
+    # so the counted warm-up and the timed launches run at sustained clocks
+    # (the reference's own method also warms up, then samples for up to
+    # 250 ms: benches/kernels.rs:41-64). Launches go in batches of about
+    # 20 ms with a synchronize after each, to read the clock.
+    per = max(cold[len(cold) // 2], 1e-3)
+    batch = max(1, int(20.0 / per))
+    pre_launches = 0
+    t_pre = time.perf_counter()
+    while (time.perf_counter() - t_pre) * 1e3 < args.preroll_ms:
+        for _ in range(batch):
+            ops.dot_product_matrix_with(Gf8D, rows.view(-1), L, M, plan, srcs)
+        pre_launches += batch
+        torch.cuda.synchronize()
+    preroll_ms = (time.perf_counter() - t_pre) * 1e3
+
+    # The counted warm-up runs back to back straight into the timed region.
     for _ in range(args.warmup):
         ops.dot_product_matrix_with(Gf8D, rows.view(-1), L, M, plan, srcs)
     torch.cuda.synchronize()
@@ -679,6 +713,14 @@ def main() -> int:
         "launch_ms": {"mean": round(avg_ms, 5), "median": round(median_ms, 5),
                       "min": round(per_launch[0], 5), "max": round(per_launch[-1], 5),
                       "host_enqueue_per_call": round(t_enqueue * 1e3 / args.steps, 5)},
+        "preroll": {"ms": round(preroll_ms, 1), "launches": pre_launches,
+                    "note": "untimed back-to-back launches before the counted warm-up, so the "
+                            "timed launches run at sustained clocks"},
+        "cold_start": {"launches": len(cold), "median_ms": round(cold[len(cold) // 2], 5),
+                       "min_ms": round(cold[0], 5), "max_ms": round(cold[-1], 5),
+                       "frac_at_median": round(bytes_per_step / (cold[len(cold) // 2] * 1e-3)
+                                               / 1e9 / HBM_PEAK_GBS, 4),
+                       "note": "first launches after an idle GPU, before the pre-roll"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(L), "kernel_avg_ms": round(avg_ms, 5),
