@@ -104,7 +104,23 @@ struct TempTable {
     }
 };
 
-int upload_temp(int f, const std::vector<uint64_t> &vals, hipStream_t s, TempTable &out) {
+// A graph capture cannot hold a per-call upload: its host staging buffer
+// and its device temporary are released after the call, so a replay would
+// read freed memory.
+int refuse_capture(const char *name, hipStream_t s) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    const hipError_t e = hipStreamIsCapturing(s, &st);
+    if (e != hipSuccess) return device_fail("hipStreamIsCapturing", e);
+    if (st != hipStreamCaptureStatusNone)
+        return fail("%s: the coefficients' tables are larger than a launch carries (%d bytes) and "
+                    "cannot be staged while the stream is being captured; prepare a Plan and use "
+                    "the _with form",
+                    name, INLINE_TAB_DW * 4);
+    return FGF_OK;
+}
+
+int upload_temp(int f, const char *name, const std::vector<uint64_t> &vals, hipStream_t s, TempTable &out) {
+    if (int rc = refuse_capture(name, s)) return rc;
     auto *host = new (std::nothrow) std::vector<uint32_t>(host_tables(f, vals.data(), vals.size()));
     if (!host) return fail("out of host memory");
     const size_t bytes = std::max<size_t>(host->size() * 4, 4);
@@ -131,19 +147,73 @@ int upload_temp(int f, const std::vector<uint64_t> &vals, hipStream_t s, TempTab
     return FGF_OK;
 }
 
-int run_matrix(int f, const uint32_t *tab, size_t stride_t, size_t stride_j, uint8_t *const *rows,
-               size_t m, const uint8_t *const *srcs, size_t k, size_t row_len, bool overwrite,
-               hipStream_t s) {
+// Where a launch's coefficient tables are: on the device (a plan's, or a
+// per-call upload) or in host memory of at most INLINE_TAB_DW dwords, which
+// the launches carry in their kernel arguments. Entry (t, j) at
+// t * stride_t + j * stride_j dwords.
+struct Tables {
+    const uint32_t *dev = nullptr;
+    const uint32_t *host = nullptr;
+    size_t stride_t = 0, stride_j = 0;
+};
+
+Tables plan_tables(const fgf_plan *p, size_t stride_t, size_t stride_j, size_t first = 0) {
+    Tables t;
+    t.dev = p->d_tab + first * entry_dw(p->nb);
+    t.stride_t = stride_t;
+    t.stride_j = stride_j;
+    return t;
+}
+
+// Where the destination rows are: a pointer array, or base + offsets[j], or
+// base + j * stride.
+struct Rows {
+    uint8_t *const *ptrs = nullptr;
+    uint8_t *base = nullptr;
+    const size_t *offsets = nullptr;
+    size_t stride = 0;
+    size_t m = 0;
+};
+
+Rows rows_ptrs(uint8_t *const *p, size_t m) {
+    Rows r;
+    r.ptrs = p;
+    r.m = m;
+    return r;
+}
+
+Rows rows_contiguous(uint8_t *base, size_t m, size_t row_len) {
+    Rows r;
+    r.base = base;
+    r.stride = row_len;
+    r.m = m;
+    return r;
+}
+
+Rows rows_at(uint8_t *base, const size_t *offsets, size_t m) {
+    Rows r;
+    r.base = base;
+    r.offsets = offsets;
+    r.m = m;
+    return r;
+}
+
+int run_matrix(int f, const Tables &tb, const Rows &rows, const uint8_t *const *srcs, size_t k,
+               size_t row_len, bool overwrite, hipStream_t s) {
     MatrixJob job;
     job.nb = field_bytes(f);
     job.fan_paar = is_fan_paar(f);
     job.srcs = srcs;
     job.k = k;
-    job.rows = rows;
-    job.m = m;
-    job.tab = tab;
-    job.tab_stride_t = stride_t;
-    job.tab_stride_j = stride_j;
+    job.rows = rows.ptrs;
+    job.row_base = rows.base;
+    job.row_offsets = rows.offsets;
+    job.row_stride = rows.stride;
+    job.m = rows.m;
+    job.tab = tb.dev;
+    job.host_tab = tb.host;
+    job.tab_stride_t = tb.stride_t;
+    job.tab_stride_j = tb.stride_j;
     job.row_len = row_len;
     job.overwrite = overwrite;
     job.stream = s;
@@ -156,20 +226,73 @@ int fill_zero(uint8_t *dst, size_t len, hipStream_t s) {
     return e == hipSuccess ? FGF_OK : device_fail("fill launch", e);
 }
 
+// Per-call tables of the one-shot forms, for n coefficients where get(i) is
+// coefficient i; entry (t, j) is coefficient t * st + j * sj. Up to
+// INLINE_TAB_DW dwords are built here, on the stack, and travel in the
+// launches' kernel arguments: nothing is allocated, nothing outlives the
+// call, and a graph capture records them with the launch. Larger tables go
+// to a stream-ordered device temporary (refused while the stream captures).
+struct CallTables {
+    uint32_t host[INLINE_TAB_DW];
+    TempTable temp;
+    Tables tb;
+};
+
+template <typename Get>
+int call_tables(int f, const char *name, size_t n, size_t st, size_t sj, Get get, hipStream_t s,
+                CallTables &ct) {
+    const size_t edw = entry_dw(field_bytes(f));
+    ct.tb.stride_t = st * edw;
+    ct.tb.stride_j = sj * edw;
+    if (n * edw <= (size_t)INLINE_TAB_DW) {
+        for (size_t i = 0; i < n; ++i) build_coeff_table(f, get(i), ct.host + i * edw);
+        ct.tb.host = ct.host;
+        return FGF_OK;
+    }
+    std::vector<uint64_t> vals(n);
+    for (size_t i = 0; i < n; ++i) vals[i] = get(i);
+    if (int rc = upload_temp(f, name, vals, s, ct.temp)) return rc;
+    ct.tb.dev = ct.temp.d;
+    return FGF_OK;
+}
+
+// Coefficient i of a packed little-endian list.
+struct FlatCoeffs {
+    const uint8_t *p;
+    int nb;
+    uint64_t operator()(size_t i) const { return read_elem(p + i * nb, nb); }
+};
+
+// Coefficient t * nrows + j of per-term lists (term t supplies nrows).
+struct TermCoeffs {
+    const void *const *terms;
+    size_t nrows;
+    int nb;
+    uint64_t operator()(size_t i) const {
+        return read_elem(static_cast<const uint8_t *>(terms[i / nrows]) + (i % nrows) * nb, nb);
+    }
+};
+
+template <typename Get>
+bool all_zero_n(size_t n, Get get) {
+    for (size_t i = 0; i < n; ++i)
+        if (get(i)) return false;
+    return true;
+}
+
 // One-shot single-coefficient op: dst (^)= c * src over len bytes.
 int single(int f, uint8_t *dst, const uint8_t *src, size_t len, uint64_t c, bool overwrite,
            hipStream_t s) {
     if (len == 0) return FGF_OK;
-    TempTable tt;
-    int rc = upload_temp(f, {c}, s, tt);
-    if (rc) return rc;
-    return run_matrix(f, tt.d, 0, 0, &dst, 1, &src, 1, len, overwrite, s);
+    CallTables ct;
+    if (int rc = call_tables(f, "mul_add", 1, 0, 0, [c](size_t) { return c; }, s, ct)) return rc;
+    return run_matrix(f, ct.tb, rows_ptrs(&dst, 1), &src, 1, len, overwrite, s);
 }
 
 int single_plan(const fgf_plan *p, size_t index, uint8_t *dst, const uint8_t *src, size_t len,
                 bool overwrite, hipStream_t s) {
     if (len == 0) return FGF_OK;
-    return run_matrix(p->field, p->d_tab + index * entry_dw(p->nb), 0, 0, &dst, 1, &src, 1, len,
+    return run_matrix(p->field, plan_tables(p, 0, 0, index), rows_ptrs(&dst, 1), &src, 1, len,
                       overwrite, s);
 }
 
@@ -265,18 +388,6 @@ bool check_matrix_terms(int f, const char *name, size_t rows_len, size_t row_len
     return true;
 }
 
-// Flatten per-term coefficient slices into (t, j) at t*nrows + j.
-bool flatten_terms(int f, const void *const *term_coeffs, size_t nterms, size_t nrows,
-                   std::vector<uint64_t> &flat, int *rc) {
-    flat.resize(nterms * nrows);
-    std::vector<uint64_t> one;
-    for (size_t t = 0; t < nterms; ++t) {
-        if (!decode_coeffs(f, term_coeffs[t], nrows, one, rc)) return false;
-        std::copy(one.begin(), one.end(), flat.begin() + t * nrows);
-    }
-    return true;
-}
-
 // Validation shared by the _with matrix forms (ops.rs:727-748, 830-851).
 bool check_matrix_plan(int f, const char *name, size_t rows_len, size_t row_len, size_t nrows,
                        const fgf_plan *plan, const size_t *src_lens, size_t nsrcs, size_t *used,
@@ -302,6 +413,15 @@ bool check_matrix_plan(int f, const char *name, size_t rows_len, size_t row_len,
             return false;
         }
     }
+    return true;
+}
+
+bool terms_present(const void *const *term_coeffs, size_t nterms, int *rc) {
+    for (size_t t = 0; t < nterms; ++t)
+        if (!term_coeffs[t]) {
+            *rc = fail("coefficient pointer is null");
+            return false;
+        }
     return true;
 }
 
@@ -576,14 +696,13 @@ int fgf_mul_add_scatter(fgf_field f, uint8_t *rows, size_t rows_len, size_t row_
         return fail("mul_add_scatter: rows is %zu bytes but %zu rows of %zu bytes need %zu", rows_len,
                     ncoeffs, row_len, used);
     if (!apart("mul_add_scatter", rows, used, &src, &src_len, 1, &rc)) return rc;
-    std::vector<uint64_t> c;
-    if (!decode_coeffs(f, coeffs, ncoeffs, c, &rc)) return rc;
-    if (all_zero(c) || row_len == 0) return FGF_OK;
-    TempTable tt;
-    if ((rc = upload_temp(f, c, S(stream), tt))) return rc;
-    auto r = contiguous_rows(rows, ncoeffs, row_len);
-    return run_matrix(f, tt.d, 0, entry_dw(field_bytes(f)), r.data(), ncoeffs, &src, 1, row_len,
-                      false, S(stream));
+    if (ncoeffs && !coeffs) return fail("coefficient pointer is null");
+    const FlatCoeffs c{static_cast<const uint8_t *>(coeffs), field_bytes(f)};
+    if (all_zero_n(ncoeffs, c) || row_len == 0) return FGF_OK;
+    CallTables ct;
+    if ((rc = call_tables(f, "mul_add_scatter", ncoeffs, 0, 1, c, S(stream), ct))) return rc;
+    return run_matrix(f, ct.tb, rows_contiguous(rows, ncoeffs, row_len), &src, 1, row_len, false,
+                      S(stream));
 }
 
 int fgf_mul_add_scatter_with(fgf_field f, uint8_t *rows, size_t rows_len, size_t row_len,
@@ -600,9 +719,8 @@ int fgf_mul_add_scatter_with(fgf_field f, uint8_t *rows, size_t rows_len, size_t
                     rows_len, n, row_len, used);
     if (!apart("mul_add_scatter_with", rows, used, &src, &src_len, 1, &rc)) return rc;
     if (all_zero(plan->values) || row_len == 0) return FGF_OK;
-    auto r = contiguous_rows(rows, n, row_len);
-    return run_matrix(f, plan->d_tab, 0, entry_dw(plan->nb), r.data(), n, &src, 1, row_len, false,
-                      S(stream));
+    return run_matrix(f, plan_tables(plan, 0, entry_dw(plan->nb)), rows_contiguous(rows, n, row_len),
+                      &src, 1, row_len, false, S(stream));
 }
 
 namespace {
@@ -635,13 +753,12 @@ int fgf_mul_add_gather(fgf_field f, uint8_t *dst, size_t dst_len, const void *co
     int rc = 0;
     if (!check_gather(f, "mul_add_gather", dst_len, ncoeffs, src_lens, nsrcs, false, false, &rc)) return rc;
     if (!apart("mul_add_gather", dst, dst_len, srcs, src_lens, nsrcs, &rc)) return rc;
-    std::vector<uint64_t> c;
-    if (!decode_coeffs(f, coeffs, ncoeffs, c, &rc)) return rc;
-    if (nsrcs == 0 || all_zero(c) || dst_len == 0) return FGF_OK;
-    TempTable tt;
-    if ((rc = upload_temp(f, c, S(stream), tt))) return rc;
-    return run_matrix(f, tt.d, entry_dw(field_bytes(f)), 0, &dst, 1, srcs, nsrcs, dst_len, false,
-                      S(stream));
+    if (ncoeffs && !coeffs) return fail("coefficient pointer is null");
+    const FlatCoeffs c{static_cast<const uint8_t *>(coeffs), field_bytes(f)};
+    if (nsrcs == 0 || all_zero_n(ncoeffs, c) || dst_len == 0) return FGF_OK;
+    CallTables ct;
+    if ((rc = call_tables(f, "mul_add_gather", ncoeffs, 1, 0, c, S(stream), ct))) return rc;
+    return run_matrix(f, ct.tb, rows_ptrs(&dst, 1), srcs, nsrcs, dst_len, false, S(stream));
 }
 
 int fgf_mul_add_gather_with(fgf_field f, uint8_t *dst, size_t dst_len, const fgf_plan *plan,
@@ -656,8 +773,8 @@ int fgf_mul_add_gather_with(fgf_field f, uint8_t *dst, size_t dst_len, const fgf
         return rc;
     if (!apart("mul_add_gather_with", dst, dst_len, srcs, src_lens, nsrcs, &rc)) return rc;
     if (nsrcs == 0 || all_zero(plan->values) || dst_len == 0) return FGF_OK;
-    return run_matrix(f, plan->d_tab, entry_dw(plan->nb), 0, &dst, 1, srcs, nsrcs, dst_len, false,
-                      S(stream));
+    return run_matrix(f, plan_tables(plan, entry_dw(plan->nb), 0), rows_ptrs(&dst, 1), srcs, nsrcs,
+                      dst_len, false, S(stream));
 }
 
 int fgf_dot_product(fgf_field f, uint8_t *dst, size_t dst_len, const void *coeffs, size_t ncoeffs,
@@ -666,15 +783,14 @@ int fgf_dot_product(fgf_field f, uint8_t *dst, size_t dst_len, const void *coeff
     int rc = 0;
     if (!check_gather(f, "dot_product", dst_len, ncoeffs, src_lens, nsrcs, true, false, &rc)) return rc;
     if (!apart("dot_product", dst, dst_len, srcs, src_lens, nsrcs, &rc)) return rc;
-    std::vector<uint64_t> c;
-    if (!decode_coeffs(f, coeffs, ncoeffs, c, &rc)) return rc;
-    if (nsrcs == 0 || all_zero(c)) return fill_zero(dst, dst_len, S(stream));
-    if (ncoeffs == 1) return fgf_mul_into(f, dst, dst_len, c[0], srcs[0], src_lens[0], stream);
+    if (ncoeffs && !coeffs) return fail("coefficient pointer is null");
+    const FlatCoeffs c{static_cast<const uint8_t *>(coeffs), field_bytes(f)};
+    if (nsrcs == 0 || all_zero_n(ncoeffs, c)) return fill_zero(dst, dst_len, S(stream));
+    if (ncoeffs == 1) return fgf_mul_into(f, dst, dst_len, c(0), srcs[0], src_lens[0], stream);
     if (dst_len == 0) return FGF_OK;
-    TempTable tt;
-    if ((rc = upload_temp(f, c, S(stream), tt))) return rc;
-    return run_matrix(f, tt.d, entry_dw(field_bytes(f)), 0, &dst, 1, srcs, nsrcs, dst_len, true,
-                      S(stream));
+    CallTables ct;
+    if ((rc = call_tables(f, "dot_product", ncoeffs, 1, 0, c, S(stream), ct))) return rc;
+    return run_matrix(f, ct.tb, rows_ptrs(&dst, 1), srcs, nsrcs, dst_len, true, S(stream));
 }
 
 int fgf_dot_product_with(fgf_field f, uint8_t *dst, size_t dst_len, const fgf_plan *plan,
@@ -692,8 +808,8 @@ int fgf_dot_product_with(fgf_field f, uint8_t *dst, size_t dst_len, const fgf_pl
     if (plan->values.size() == 1)
         return fgf_mul_into_with(f, dst, dst_len, plan, 0, srcs[0], src_lens[0], stream);
     if (dst_len == 0) return FGF_OK;
-    return run_matrix(f, plan->d_tab, entry_dw(plan->nb), 0, &dst, 1, srcs, nsrcs, dst_len, true,
-                      S(stream));
+    return run_matrix(f, plan_tables(plan, entry_dw(plan->nb), 0), rows_ptrs(&dst, 1), srcs, nsrcs,
+                      dst_len, true, S(stream));
 }
 
 // ---------------------------------------------------------------------------
@@ -712,14 +828,13 @@ int fgf_mul_add_matrix(fgf_field f, uint8_t *rows, size_t rows_len, size_t row_l
         return rc;
     if (!apart("mul_add_matrix", rows, used, term_srcs, term_src_lens, nterms, &rc)) return rc;
     if (nrows == 0 || nterms == 0 || row_len == 0) return FGF_OK;
-    std::vector<uint64_t> flat;
-    if (!flatten_terms(f, term_coeffs, nterms, nrows, flat, &rc)) return rc;
-    TempTable tt;
-    if ((rc = upload_temp(f, flat, S(stream), tt))) return rc;
-    const size_t e = entry_dw(field_bytes(f));
-    auto r = contiguous_rows(rows, nrows, row_len);
-    return run_matrix(f, tt.d, nrows * e, e, r.data(), nrows, term_srcs, nterms, row_len, false,
-                      S(stream));
+    if (!terms_present(term_coeffs, nterms, &rc)) return rc;
+    CallTables ct;
+    if ((rc = call_tables(f, "mul_add_matrix", nterms * nrows, nrows, 1, TermCoeffs{term_coeffs, nrows, field_bytes(f)},
+                          S(stream), ct)))
+        return rc;
+    return run_matrix(f, ct.tb, rows_contiguous(rows, nrows, row_len), term_srcs, nterms, row_len,
+                      false, S(stream));
 }
 
 int fgf_mul_add_matrix_with(fgf_field f, uint8_t *rows, size_t rows_len, size_t row_len,
@@ -734,9 +849,8 @@ int fgf_mul_add_matrix_with(fgf_field f, uint8_t *rows, size_t rows_len, size_t 
     if (!apart("mul_add_matrix_with", rows, used, srcs, src_lens, nsrcs, &rc)) return rc;
     if (nrows == 0 || nsrcs == 0 || row_len == 0) return FGF_OK;
     const size_t e = entry_dw(plan->nb);
-    auto r = contiguous_rows(rows, nrows, row_len);
-    return run_matrix(f, plan->d_tab, nrows * e, e, r.data(), nrows, srcs, nsrcs, row_len, false,
-                      S(stream));
+    return run_matrix(f, plan_tables(plan, nrows * e, e), rows_contiguous(rows, nrows, row_len), srcs,
+                      nsrcs, row_len, false, S(stream));
 }
 
 int fgf_dot_product_matrix(fgf_field f, uint8_t *rows, size_t rows_len, size_t row_len,
@@ -753,14 +867,13 @@ int fgf_dot_product_matrix(fgf_field f, uint8_t *rows, size_t rows_len, size_t r
     if (nrows == 0) return FGF_OK;
     if (nterms == 0) return fill_zero(rows, used, S(stream));
     if (row_len == 0) return FGF_OK;
-    std::vector<uint64_t> flat;
-    if (!flatten_terms(f, term_coeffs, nterms, nrows, flat, &rc)) return rc;
-    TempTable tt;
-    if ((rc = upload_temp(f, flat, S(stream), tt))) return rc;
-    const size_t e = entry_dw(field_bytes(f));
-    auto r = contiguous_rows(rows, nrows, row_len);
-    return run_matrix(f, tt.d, nrows * e, e, r.data(), nrows, term_srcs, nterms, row_len, true,
-                      S(stream));
+    if (!terms_present(term_coeffs, nterms, &rc)) return rc;
+    CallTables ct;
+    if ((rc = call_tables(f, "dot_product_matrix", nterms * nrows, nrows, 1, TermCoeffs{term_coeffs, nrows, field_bytes(f)},
+                          S(stream), ct)))
+        return rc;
+    return run_matrix(f, ct.tb, rows_contiguous(rows, nrows, row_len), term_srcs, nterms, row_len,
+                      true, S(stream));
 }
 
 int fgf_dot_product_matrix_with(fgf_field f, uint8_t *rows, size_t rows_len, size_t row_len,
@@ -777,9 +890,8 @@ int fgf_dot_product_matrix_with(fgf_field f, uint8_t *rows, size_t rows_len, siz
     if (nsrcs == 0) return fill_zero(rows, used, S(stream));
     if (row_len == 0) return FGF_OK;
     const size_t e = entry_dw(plan->nb);
-    auto r = contiguous_rows(rows, nrows, row_len);
-    return run_matrix(f, plan->d_tab, nrows * e, e, r.data(), nrows, srcs, nsrcs, row_len, true,
-                      S(stream));
+    return run_matrix(f, plan_tables(plan, nrows * e, e), rows_contiguous(rows, nrows, row_len), srcs,
+                      nsrcs, row_len, true, S(stream));
 }
 
 namespace {
@@ -902,7 +1014,8 @@ int host_pipeline(int f, const char *name, const std::vector<uint8_t *> &hrows, 
         if ((e = hipEventRecord(p.ev[0][b], up)) != hipSuccess ||
             (e = hipStreamWaitEvent(enc, p.ev[0][b], 0)) != hipSuccess)
             return device_fail("host pipeline ordering", e);
-        if ((rc = run_matrix(f, plan->d_tab, m * edw, edw, drow.data(), m, dsrc.data(), k, c, overwrite, enc)))
+        if ((rc = run_matrix(f, plan_tables(plan, m * edw, edw), rows_ptrs(drow.data(), m), dsrc.data(),
+                             k, c, overwrite, enc)))
             return rc;
         if ((e = hipEventRecord(p.ev[1][b], enc)) != hipSuccess ||
             (e = hipStreamWaitEvent(down, p.ev[1][b], 0)) != hipSuccess)
@@ -1049,14 +1162,12 @@ int fgf_mul_add_matrix_scattered(fgf_field f, uint8_t *dst, size_t dst_len, size
     if (!rows_apart(name, dst, row_starts, nrows, row_len, term_srcs, term_src_lens, nterms, &rc))
         return rc;
     if (nrows == 0 || nterms == 0 || row_len == 0) return FGF_OK;
-    std::vector<uint64_t> flat;
-    if (!flatten_terms(f, term_coeffs, nterms, nrows, flat, &rc)) return rc;
-    TempTable tt;
-    if ((rc = upload_temp(f, flat, S(stream), tt))) return rc;
-    std::vector<uint8_t *> r(nrows);
-    for (size_t j = 0; j < nrows; ++j) r[j] = dst + row_starts[j];
-    const size_t e = entry_dw(field_bytes(f));
-    return run_matrix(f, tt.d, nrows * e, e, r.data(), nrows, term_srcs, nterms, row_len, false,
+    if (!terms_present(term_coeffs, nterms, &rc)) return rc;
+    CallTables ct;
+    if ((rc = call_tables(f, "mul_add_matrix_scattered", nterms * nrows, nrows, 1, TermCoeffs{term_coeffs, nrows, field_bytes(f)},
+                          S(stream), ct)))
+        return rc;
+    return run_matrix(f, ct.tb, rows_at(dst, row_starts, nrows), term_srcs, nterms, row_len, false,
                       S(stream));
 }
 
@@ -1085,10 +1196,9 @@ int scattered_with(int f, const char *name, uint8_t *dst, size_t dst_len, size_t
             if ((rc = fill_zero(dst + row_starts[j], row_len, s))) return rc;
         return FGF_OK;
     }
-    std::vector<uint8_t *> r(nrows);
-    for (size_t j = 0; j < nrows; ++j) r[j] = dst + row_starts[j];
     const size_t e = entry_dw(plan->nb);
-    return run_matrix(f, plan->d_tab, nrows * e, e, r.data(), nrows, srcs, nsrcs, row_len, overwrite, s);
+    return run_matrix(f, plan_tables(plan, nrows * e, e), rows_at(dst, row_starts, nrows), srcs, nsrcs,
+                      row_len, overwrite, s);
 }
 
 }  // namespace

@@ -278,6 +278,48 @@ int batch_build_items(fgf_batch *b) {
 
 }  // namespace
 
+namespace {
+
+// Stripes run concurrently in one launch, so no stripe's row may overlap
+// another stripe's row or source (sources may be shared between stripes:
+// they are only read). Rows sorted by address: any overlap among them shows
+// between neighbours, and once they are disjoint their ends are sorted too,
+// so a source overlaps some row exactly when it overlaps the last row that
+// starts before the source ends. O(N log N) in the number of shards.
+int check_stripes_apart(const fgf_stripe *stripes, size_t n) {
+    struct Span {
+        uintptr_t lo, hi;
+        size_t stripe, index;
+    };
+    std::vector<Span> rows;
+    for (size_t s = 0; s < n; ++s)
+        if (stripes[s].row_len)
+            for (size_t j = 0; j < stripes[s].plan->cols; ++j) {
+                const uintptr_t lo = (uintptr_t)stripes[s].rows[j];
+                rows.push_back({lo, lo + stripes[s].row_len, s, j});
+            }
+    std::sort(rows.begin(), rows.end(), [](const Span &a, const Span &b) { return a.lo < b.lo; });
+    for (size_t i = 1; i < rows.size(); ++i)
+        if (rows[i].lo < rows[i - 1].hi)
+            return bfail("batch stripes %zu and %zu: row %zu of stripe %zu overlaps row %zu of stripe %zu",
+                         rows[i - 1].stripe, rows[i].stripe, rows[i].index, rows[i].stripe,
+                         rows[i - 1].index, rows[i - 1].stripe);
+    for (size_t s = 0; s < n; ++s) {
+        if (!stripes[s].row_len) continue;
+        for (size_t t = 0; t < stripes[s].plan->rows; ++t) {
+            const uintptr_t lo = (uintptr_t)stripes[s].srcs[t], hi = lo + stripes[s].row_len;
+            auto it = std::lower_bound(rows.begin(), rows.end(), hi,
+                                       [](const Span &a, uintptr_t v) { return a.lo < v; });
+            if (it != rows.begin() && (it - 1)->hi > lo)
+                return bfail("batch stripes %zu and %zu: source %zu of stripe %zu overlaps row %zu of stripe %zu",
+                             s, (it - 1)->stripe, t, s, (it - 1)->index, (it - 1)->stripe);
+        }
+    }
+    return FGF_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int fgf_batch_create(fgf_field f, const fgf_stripe *stripes, size_t n, fgf_batch **out) {
@@ -318,6 +360,7 @@ int fgf_batch_create(fgf_field f, const fgf_stripe *stripes, size_t n, fgf_batch
             }
         nptrs += p->rows + p->cols;
     }
+    if (int rc = check_stripes_apart(stripes, n)) return rc;
     auto *b = new (std::nothrow) fgf_batch;
     if (!b) return bfail("out of host memory");
     b->field = f;

@@ -52,7 +52,7 @@ using namespace core;
 struct MatrixArgs {
     const uint8_t *src[KMAX];
     uint8_t *dst[MMAX];
-    const uint32_t *tab;     // entry (t = 0, j = 0) of this launch
+    const uint32_t *tab;     // entry (t = 0, j = 0) of this launch; null: itab
     uint64_t col0;           // first byte column of the segment
     uint64_t ncols;          // bytes in the segment
     uint32_t k;              // sources in this launch
@@ -61,7 +61,12 @@ struct MatrixArgs {
     uint32_t tab_stride_j;   // dwords
     uint32_t ngroups;        // row groups of M rows
     uint32_t ncoltiles;      // column tiles (vector kernel) or groups of 4 elements (generic)
+    // The launch's tables in its kernel arguments (when tab is null): read
+    // in place from the kernarg segment, like any other argument.
+    uint32_t itab[INLINE_TAB_DW];
 };
+
+__device__ __forceinline__ const uint32_t *table_of(const MatrixArgs &a) { return a.tab ? a.tab : a.itab; }
 
 constexpr int BLOCK = 256;
 
@@ -118,7 +123,7 @@ void k_matrix(const MatrixArgs a) {
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint64_t tile = ((uint64_t)ct * (BLOCK / 64) + wave) * (256u * D);
     const uint32_t j0 = rg * M;
-    const uint32_t *tab = a.tab + (size_t)j0 * a.tab_stride_j;
+    const uint32_t *tab = table_of(a) + (size_t)j0 * a.tab_stride_j;
     extern __shared__ uint4 s_lt[];
     const auto tb = stage_lds_table<B, t2_lds(B, M)>(s_lt, tab, a.k, M, a.tab_stride_t,
                                                   a.tab_stride_j);
@@ -155,7 +160,7 @@ __global__ __launch_bounds__(BLOCK) void k_matrix_generic(const MatrixArgs a) {
 #pragma unroll
             for (int q = 0; q < B; ++q) base[q] = selectors(pl[q]);
             kara_expand<B, FP>(base, sel);
-            const uint32_t *tj = a.tab + (size_t)t * a.tab_stride_t + (size_t)j * a.tab_stride_j;
+            const uint32_t *tj = table_of(a) + (size_t)t * a.tab_stride_t + (size_t)j * a.tab_stride_j;
             if constexpr (B == 1) {
                 acc[0] = cmul_acc(acc[0], tj, sel[0]);
             } else {
@@ -253,10 +258,25 @@ hipError_t launch_segment(const MatrixJob &job, MatrixArgs a, size_t k0, size_t 
                           size_t mn, uint64_t col0, uint64_t ncols, bool vector, bool ow) {
     if (ncols == 0) return hipSuccess;
     for (size_t t = 0; t < kn; ++t) a.src[t] = job.srcs[k0 + t];
-    for (size_t j = 0; j < mn; ++j) a.dst[j] = job.rows[j0 + j];
-    a.tab = job.tab + k0 * job.tab_stride_t + j0 * job.tab_stride_j;
-    a.tab_stride_t = (uint32_t)job.tab_stride_t;
-    a.tab_stride_j = (uint32_t)job.tab_stride_j;
+    for (size_t j = 0; j < mn; ++j) a.dst[j] = job.row(j0 + j);
+    if (job.tab) {
+        a.tab = job.tab + k0 * job.tab_stride_t + j0 * job.tab_stride_j;
+        a.tab_stride_t = (uint32_t)job.tab_stride_t;
+        a.tab_stride_j = (uint32_t)job.tab_stride_j;
+    } else {
+        // The entries of sources [k0, k0 + kn) and rows [j0, j0 + mn), packed
+        // (t * mn + j) into the kernel arguments.
+        const size_t e = entry_dwords(job.nb);
+        if (kn * mn * e > (size_t)INLINE_TAB_DW) return hipErrorInvalidValue;
+        for (size_t t = 0; t < kn; ++t)
+            for (size_t j = 0; j < mn; ++j)
+                std::memcpy(a.itab + (t * mn + j) * e,
+                            job.host_tab + (k0 + t) * job.tab_stride_t + (j0 + j) * job.tab_stride_j,
+                            e * sizeof(uint32_t));
+        a.tab = nullptr;
+        a.tab_stride_t = (uint32_t)(mn * e);
+        a.tab_stride_j = (uint32_t)e;
+    }
     a.k = (uint32_t)kn;
     a.m = (uint32_t)mn;
     a.col0 = col0;
@@ -307,7 +327,7 @@ hipError_t launch_matrix(const MatrixJob &job) {
     const uintptr_t mis = (uintptr_t)job.srcs[0] & 15u;
     bool same = true;
     for (size_t t = 0; t < job.k; ++t) same &= (((uintptr_t)job.srcs[t] & 15u) == mis);
-    for (size_t j = 0; j < job.m; ++j) same &= (((uintptr_t)job.rows[j] & 15u) == mis);
+    for (size_t j = 0; j < job.m; ++j) same &= (((uintptr_t)job.row(j) & 15u) == mis);
     uint64_t head = (16u - mis) & 15u;
     uint64_t body = 0;
     if (same && head % nb == 0 && head <= job.row_len) body = ((job.row_len - head) / 16u) * 16u;

@@ -25,23 +25,44 @@ constexpr int kterms(int nb) { return nb == 1 ? 1 : nb == 2 ? 3 : nb == 4 ? 9 : 
 // Dwords of one coefficient's tables.
 constexpr size_t entry_dwords(int nb) { return (size_t)kterms(nb) * TAB_DW; }
 
+// Dwords of coefficient tables a launch can carry inside its kernel
+// arguments (1 KiB: the 40 byte constants of RS(10+4), or 8 GF(2^16)
+// coefficients). A per-call table this small needs no device allocation.
+constexpr int INLINE_TAB_DW = 256;
+
 // One matrix operation: rows[j] (op)= sum_t c(t, j) * srcs[t] over row_len
 // bytes. The table entry of coefficient (t, j) starts at
 // tab + t * tab_stride_t + j * tab_stride_j (dwords) and holds the kterms(B)
-// byte-constant tables that build_coeff_table lays out.
+// byte-constant tables that build_coeff_table lays out. The table is either
+// on the device (tab) or, when it holds at most INLINE_TAB_DW dwords, in
+// host memory (host_tab), from where each launch copies the entries it
+// needs into its kernel arguments.
+//
+// Row j is rows[j] when a pointer array is given, else
+// row_base + row_offsets[j] when offsets are given, else
+// row_base + j * row_stride: the contiguous and scattered shapes need no
+// per-call pointer array.
 struct MatrixJob {
     int nb = 1;                          // field bytes B
     bool fan_paar = false;               // Fan-Paar tower (Karatsuba branch order)
     const uint8_t *const *srcs = nullptr;  // host array of k device pointers
     size_t k = 0;
-    uint8_t *const *rows = nullptr;        // host array of m device pointers
+    uint8_t *const *rows = nullptr;        // host array of m device pointers, or
+    uint8_t *row_base = nullptr;           // device base of the rows
+    const size_t *row_offsets = nullptr;   // host array of m byte offsets from row_base
+    size_t row_stride = 0;                 // bytes between rows (no offsets)
     size_t m = 0;
-    const uint32_t *tab = nullptr;         // device
+    const uint32_t *tab = nullptr;         // device, or
+    const uint32_t *host_tab = nullptr;    // host (at most INLINE_TAB_DW dwords)
     size_t tab_stride_t = 0;
     size_t tab_stride_j = 0;
     size_t row_len = 0;
     bool overwrite = false;
     hipStream_t stream = nullptr;
+
+    uint8_t *row(size_t j) const {
+        return rows ? rows[j] : row_base + (row_offsets ? row_offsets[j] : j * row_stride);
+    }
 };
 
 hipError_t launch_matrix(const MatrixJob &job);

@@ -42,8 +42,14 @@ class Buf:
 
 
 def _buf(x) -> tuple[int, int]:
+    """A device byte range: a Buf, or a contiguous tensor on the GPU. Host
+    memory is refused before anything is enqueued: a kernel given a host
+    address would fault."""
     if isinstance(x, Buf):
         return x.ptr, x.nbytes
+    if not getattr(x, "is_cuda", False):
+        raise Panic("device form takes device memory, got a host buffer "
+                    "(use a *_with_host form for host memory)")
     if not x.is_contiguous():
         raise Panic("buffer must be a contiguous byte range")
     return x.data_ptr(), x.nbytes
@@ -86,6 +92,9 @@ def _ptr_array(bufs: Sequence) -> tuple[ctypes.Array, ctypes.Array]:
             ptrs.append(b.ptr)
             lens.append(b.nbytes)
             continue
+        if not getattr(b, "is_cuda", False):
+            raise Panic("device form takes device memory, got a host buffer "
+                        "(use a *_with_host form for host memory)")
         if not b.is_contiguous():
             raise Panic("buffer must be a contiguous byte range")
         ptrs.append(b.data_ptr())
@@ -346,7 +355,9 @@ def _hbuf(x) -> tuple[int, int]:
         return x.ctypes.data, x.nbytes
     if getattr(x, "is_cuda", False):
         raise Panic("host-buffer form takes host memory, got a device tensor")
-    return _buf(x)
+    if not x.is_contiguous():
+        raise Panic("buffer must be a contiguous byte range")
+    return x.data_ptr(), x.nbytes
 
 
 def _host_matrix(fn, field: Field, rows, row_len: int, nrows: int, plan: Plan, srcs: Sequence,

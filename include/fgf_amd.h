@@ -25,8 +25,22 @@
  *    borrow rules): a source overlapping a destination row is refused with
  *    FGF_EINVAL ("<op>: source i overlaps the destination"). Sources may
  *    alias each other. `fgf_mul_assign` is in place by definition.
- *  - The `_with` forms allocate nothing per call. Plans hold their tables in
- *    device memory on the device that was current when they were built.
+ *  - Host allocations per call (the reference's steady state allocates
+ *    nothing, tests/zero_alloc.rs:81-148): the device-buffer `_with` forms
+ *    allocate nothing. The one-shot forms build their coefficient tables on
+ *    the stack and pass them inside the launches' kernel arguments when they
+ *    fit in 1 KiB (256 dwords: entry_dwords = 5 per byte field coefficient,
+ *    15/45/135 for GF(2^16/32/64)), and then allocate nothing either; larger
+ *    per-call tables go to a stream-ordered device temporary, staged from a
+ *    host buffer that a host callback frees. The host-buffer forms
+ *    (`*_with_host`) keep small host arrays per call.
+ *  - Graph capture: the `_with` forms and the one-shot forms whose tables
+ *    fit the kernel arguments can be captured into a hipGraph and replayed.
+ *    A one-shot call whose tables need the device temporary is refused with
+ *    FGF_EINVAL while its stream is capturing (prepare a Plan instead).
+ *  - Plans hold their tables in device memory on the device that was
+ *    current when they were built. They are immutable: any number of
+ *    streams and threads may use one plan at once.
  */
 #ifndef FGF_AMD_H
 #define FGF_AMD_H
@@ -260,8 +274,11 @@ int fgf_coeff_table(fgf_field field, uint64_t coeff, uint32_t *out);
  * fgf_dot_product_matrix_with (overwrite = 1) or fgf_mul_add_matrix_with
  * (overwrite = 0) computes for the same arguments. Requirements beyond those
  * calls: row_len a multiple of 16, every pointer 16-byte aligned, plan
- * dimensions (k, m) both nonzero. Plans and buffers must outlive the batch;
- * distinct stripes must not write overlapping rows.
+ * dimensions (k, m) both nonzero. Plans and buffers must outlive the batch.
+ * Stripes run concurrently, so creation refuses a row that overlaps any
+ * other row or any source, of its own stripe or another (sources may be
+ * shared: they are only read). A batch holds one work counter: one batch
+ * must not run on two streams at once (distinct batches may).
  * ---------------------------------------------------------------------- */
 typedef struct fgf_batch fgf_batch;
 typedef struct fgf_stripe {
@@ -305,6 +322,10 @@ void fgf_batch_free(fgf_batch *batch);
  * into one persistent launch per free slot (each slot its own stream, so
  * batches overlap on the device), and keeps coefficient plans — encode
  * matrices and inverted decode matrices — in a device LRU cache.
+ * Requests in flight together run concurrently: a request's shards must not
+ * overlap another in-flight request's shards that either of them writes
+ * (parity on encode, the erased shards on reconstruct). Within a request
+ * every shard must be distinct; that is checked at submission.
  * ---------------------------------------------------------------------- */
 typedef struct fgf_server fgf_server;
 typedef struct fgf_server_config {

@@ -216,28 +216,50 @@ struct Term {
 namespace ops {
 
 namespace detail {
+/* Per-call pointer and length arrays for the C ABI: inline for up to N
+ * entries, so a call with at most 64 sources allocates nothing (the
+ * reference's steady state allocates nothing, tests/zero_alloc.rs). */
+template <class T, std::size_t N = 64>
+class SmallBuf {
+  public:
+    explicit SmallBuf(std::size_t n) : p_(n <= N ? inline_ : grow(n)) {}
+    SmallBuf(const SmallBuf &) = delete;
+    SmallBuf &operator=(const SmallBuf &) = delete;
+    T &operator[](std::size_t i) { return p_[i]; }
+    T *data() { return p_; }
+
+  private:
+    T *grow(std::size_t n) {
+        heap_.resize(n);
+        return heap_.data();
+    }
+    T inline_[N];
+    std::vector<T> heap_;
+    T *p_;
+};
 struct Srcs {
-    std::vector<const uint8_t *> ptrs;
-    std::vector<std::size_t> lens;
-    explicit Srcs(const std::vector<CBytes> &s) {
-        for (const CBytes &b : s) {
-            ptrs.push_back(b.ptr);
-            lens.push_back(b.len);
+    SmallBuf<const uint8_t *> ptrs;
+    SmallBuf<std::size_t> lens;
+    explicit Srcs(const std::vector<CBytes> &s) : ptrs(s.size()), lens(s.size()) {
+        for (std::size_t i = 0; i < s.size(); ++i) {
+            ptrs[i] = s[i].ptr;
+            lens[i] = s[i].len;
         }
     }
 };
 template <class F>
 struct Terms {
-    std::vector<const void *> coeffs;
-    std::vector<std::size_t> ncoeffs;
-    std::vector<const uint8_t *> srcs;
-    std::vector<std::size_t> lens;
-    explicit Terms(const std::vector<Term<F>> &t) {
-        for (const Term<F> &x : t) {
-            coeffs.push_back(x.coeffs.data());
-            ncoeffs.push_back(x.coeffs.size());
-            srcs.push_back(x.src.ptr);
-            lens.push_back(x.src.len);
+    SmallBuf<const void *> coeffs;
+    SmallBuf<std::size_t> ncoeffs;
+    SmallBuf<const uint8_t *> srcs;
+    SmallBuf<std::size_t> lens;
+    explicit Terms(const std::vector<Term<F>> &t)
+        : coeffs(t.size()), ncoeffs(t.size()), srcs(t.size()), lens(t.size()) {
+        for (std::size_t i = 0; i < t.size(); ++i) {
+            coeffs[i] = t[i].coeffs.data();
+            ncoeffs[i] = t[i].coeffs.size();
+            srcs[i] = t[i].src.ptr;
+            lens[i] = t[i].src.len;
         }
     }
 };

@@ -207,8 +207,62 @@ def test_pack_unpack_round_trip():
 # in for device buffers: every call below fails before touching them.
 # ---------------------------------------------------------------------------
 
+class HostStandIn:
+    """A CPU tensor presented as device memory. The checks below fail on the
+    host before anything is enqueued, so nothing ever dereferences it."""
+
+    is_cuda = True
+
+    def __init__(self, t):
+        self.t = t
+
+    def __getitem__(self, s):
+        return HostStandIn(self.t[s])
+
+    def is_contiguous(self):
+        return self.t.is_contiguous()
+
+    def data_ptr(self):
+        return self.t.data_ptr()
+
+    @property
+    def nbytes(self):
+        return self.t.nbytes
+
+    def get_device(self):
+        return 0
+
+
+@pytest.fixture(autouse=True)
+def _null_stream(monkeypatch):
+    # No GPU here: stand-in buffers report the null stream.
+    monkeypatch.setattr(ops, "_current_stream", lambda index: 0)
+
+
 def z(n):
-    return torch.zeros(n, dtype=torch.uint8)
+    return HostStandIn(torch.zeros(n, dtype=torch.uint8))
+
+
+def test_device_forms_refuse_host_tensors(monkeypatch):
+    # A host pointer handed to a kernel would fault the GPU; the mirror
+    # refuses it before calling into the library at all.
+    def no_call():
+        raise AssertionError("the library was called with a host buffer")
+    monkeypatch.setattr(ops._lib, "lib", no_call)
+    host = torch.zeros(64, dtype=torch.uint8)
+    msg = "device form takes device memory"
+    with pytest.raises(Panic, match=msg):
+        ops.mul_add(Gf8B, host[:32], 3, host[32:])
+    with pytest.raises(Panic, match=msg):
+        ops.mul_add(Gf8B, z(32), 3, host[32:])
+    plan = ops.Plan(Gf8B, 0)
+    plan._h = ctypes.c_void_p(None)  # never reaches the library
+    with pytest.raises(Panic, match=msg):
+        ops.dot_product_matrix_with(Gf8B, host[:32], 16, 2, plan, [z(16)])
+    with pytest.raises(Panic, match=msg):
+        ops.dot_product_matrix_with(Gf8B, z(32), 16, 2, plan, [z(16), host[:16]])
+    with pytest.raises(Panic, match=msg):
+        ops.mul_elementwise(Gf8D, z(8), host[:8], z(8))
 
 
 def test_panic_mul_add_length_mismatch():

@@ -182,3 +182,21 @@ def test_batch_rejects_misaligned_and_overlapping_rows():
         Batch(Gf8D, [Stripe(plan, [buf[:64]], [buf[1024:1088], buf[1056:1120]], 64)])
     with pytest.raises(Panic, match="source 0 overlaps row 1"):
         Batch(Gf8D, [Stripe(plan, [buf[2048:2112]], [buf[1024:1088], buf[2080:2144]], 64)])
+
+
+def test_batch_refuses_stripes_that_overlap_each_other():
+    # Stripes run concurrently in one launch: a row overlapping another
+    # stripe's row or source would race. Shared sources are fine.
+    plan = ops.Plan.matrix(Gf8D, 2, 2, [3, 5, 7, 9])
+    buf = torch.zeros(16 << 10, dtype=torch.uint8, device="cuda")
+    L = 1024
+    seg = lambda i: buf[i * L:(i + 1) * L]
+    srcs = [seg(0), seg(1)]
+    ok = [Stripe(plan, srcs, [seg(2), seg(3)], L), Stripe(plan, srcs, [seg(4), seg(5)], L)]
+    Batch(Gf8D, ok)
+    with pytest.raises(Panic, match="batch stripes 0 and 1: row 0 of stripe 1 overlaps row 1 of stripe 0"):
+        Batch(Gf8D, [Stripe(plan, srcs, [seg(2), seg(3)], L),
+                     Stripe(plan, srcs, [buf[3 * L + 512:4 * L + 512], seg(6)], L)])
+    with pytest.raises(Panic, match="source 1 of stripe 1 overlaps row 0 of stripe 0"):
+        Batch(Gf8D, [Stripe(plan, srcs, [seg(2), seg(3)], L),
+                     Stripe(plan, [seg(0), seg(2)], [seg(4), seg(5)], L)])

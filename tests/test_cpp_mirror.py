@@ -1,37 +1,61 @@
-"""The C++ mirror of `fgf::ops` (include/fgf_amd.hpp) and its parity tests
-(tests/cpp/ops_parity.cpp, named after tests/ops.rs), run as one binary.
+"""The C++ mirror of `fgf::ops` (include/fgf_amd.hpp) and its tests, run as
+binaries: tests/cpp/ops_parity.cpp (parity, named after tests/ops.rs) and
+tests/cpp/zero_alloc.cpp (steady-state host allocations, after
+tests/zero_alloc.rs).
 
 The CPU test runs the misuse cases, which fail validation before any device
-access; the GPU test runs every case against the oracle on a gfx950 device.
+access; the GPU tests run every parity case against the oracle on a gfx950
+device and count the allocations of steady-state calls.
 """
 import os
+import re
 import subprocess
 
 import pytest
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-BIN = os.path.join(HERE, "cpp", "ops_parity")
 
 
-def _binary() -> str:
-    if not os.path.exists(BIN):
-        subprocess.run(["make", "-s", "-C", os.path.join(HERE, "cpp")], check=True)
-    return BIN
+def _binary(name: str) -> str:
+    path = os.path.join(HERE, "cpp", name)
+    if not os.path.exists(path):
+        subprocess.run(["make", "-s", "-C", os.path.join(HERE, "cpp"), name], check=True)
+    return path
 
 
-def _run(args, timeout):
-    r = subprocess.run([_binary()] + args, capture_output=True, text=True, timeout=timeout)
+def _run(name, args, timeout):
+    r = subprocess.run([_binary(name)] + args, capture_output=True, text=True, timeout=timeout)
     print(r.stdout)
     assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-2000:]
     return r.stdout
 
 
+def _all_passed(out: str) -> int:
+    """The binary's summary line "P of N tests passed" with P == N > 0."""
+    m = re.findall(r"^(\d+) of (\d+) tests passed$", out, re.M)
+    assert m, "no summary line in the output"
+    passed, ran = (int(x) for x in m[-1])
+    assert ran > 0 and passed == ran, m[-1]
+    return ran
+
+
 def test_cpp_mirror_misuse_panics_with_reference_text():
-    out = _run(["--cpu"], 60)
-    assert "12 of 12 tests passed" in out
+    out = _run("ops_parity", ["--cpu"], 60)
+    assert _all_passed(out) == 12
 
 
 @pytest.mark.gpu
 def test_cpp_mirror_matches_oracle_on_device():
-    out = _run([], 110)
-    assert " 0 of " not in out and "FAILED" not in out
+    out = _run("ops_parity", [], 110)
+    assert _all_passed(out) > 12
+    assert "FAILED" not in out
+
+
+@pytest.mark.gpu
+def test_cpp_mirror_steady_state_allocates_nothing():
+    # One-shot and prepared dot_product and dot_product_matrix after a
+    # warm-up call: no allocation by the library or the mirror
+    # (tests/zero_alloc.rs:81-148), outputs bit-exact with the oracle.
+    out = _run("zero_alloc", [], 60)
+    assert _all_passed(out) == 4
+    assert out.count("library allocations 0,") == 4
