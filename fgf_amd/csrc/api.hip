@@ -22,27 +22,35 @@ namespace {
 
 using namespace fgf;
 
-thread_local std::string g_err;
+// The last error's text, per thread, in a fixed buffer: reporting an error
+// allocates nothing either.
+thread_local char g_err[512];
+
+void set_errf(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+void set_errf(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
 
 int fail(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
 int fail(const char *fmt, ...) {
-    char buf[512];
     va_list ap;
     va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
     va_end(ap);
-    g_err = buf;
     return FGF_EINVAL;
 }
 
 }  // namespace
 
-void fgf::set_last_error(const char *msg) { g_err = msg; }
+void fgf::set_last_error(const char *msg) { set_errf("%s", msg); }
 
 namespace {
 
 int device_fail(const char *what, hipError_t e) {
-    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    set_errf("%s: %s", what, hipGetErrorString(e));
     return FGF_EDEVICE;
 }
 
@@ -128,7 +136,7 @@ int upload_temp(int f, const char *name, const std::vector<uint64_t> &vals, hipS
     if (e != hipSuccess) {
         delete host;
         out.d = nullptr;
-        g_err = std::string("hipMallocAsync: ") + hipGetErrorString(e);
+        set_errf("hipMallocAsync: %s", hipGetErrorString(e));
         return FGF_ENOMEM;
     }
     out.s = s;
@@ -449,7 +457,7 @@ int make_plan(int f, size_t rows, size_t cols, const std::vector<uint64_t> &vals
     hipError_t e = hipMalloc((void **)&p->d_tab, bytes);
     if (e != hipSuccess) {
         delete p;
-        g_err = std::string("hipMalloc: ") + hipGetErrorString(e);
+        set_errf("hipMalloc: %s", hipGetErrorString(e));
         return FGF_ENOMEM;
     }
     if (!tab.empty() && (e = hipMemcpy(p->d_tab, tab.data(), tab.size() * 4, hipMemcpyHostToDevice)) != hipSuccess) {
@@ -474,7 +482,7 @@ bool coeff_in_field(int f, uint64_t c, const char *name, int *rc) {
 
 extern "C" {
 
-const char *fgf_last_error(void) { return g_err.c_str(); }
+const char *fgf_last_error(void) { return g_err; }
 
 const char *fgf_backend(void) { return "hip-gfx950"; }
 
@@ -1144,6 +1152,30 @@ bool check_scattered_rows(int f, const char *name, size_t dst_len, size_t row_le
 
 }  // namespace
 
+namespace {
+
+// Validation of the one-shot scattered forms (ops.rs:887-941): rows, then
+// per term its coefficient count and source, then sources apart from rows.
+bool check_scattered_terms(int f, const char *name, uint8_t *dst, size_t dst_len, size_t row_len,
+                           const size_t *row_starts, size_t nrows, const size_t *term_ncoeffs,
+                           const uint8_t *const *term_srcs, const size_t *term_src_lens,
+                           size_t nterms, int *rc) {
+    if (!check_scattered_rows(f, name, dst_len, row_len, row_starts, nrows, rc)) return false;
+    for (size_t t = 0; t < nterms; ++t) {
+        if (term_ncoeffs[t] != nrows) {
+            *rc = fail("%s: term supplies %zu coefficients for %zu rows", name, term_ncoeffs[t], nrows);
+            return false;
+        }
+        if (term_src_lens[t] != row_len) {
+            *rc = fail("%s: source is %zu bytes, expected %zu", name, term_src_lens[t], row_len);
+            return false;
+        }
+    }
+    return rows_apart(name, dst, row_starts, nrows, row_len, term_srcs, term_src_lens, nterms, rc);
+}
+
+}  // namespace
+
 int fgf_mul_add_matrix_scattered(fgf_field f, uint8_t *dst, size_t dst_len, size_t row_len,
                                  const size_t *row_starts, size_t nrows,
                                  const void *const *term_coeffs, const size_t *term_ncoeffs,
@@ -1152,19 +1184,13 @@ int fgf_mul_add_matrix_scattered(fgf_field f, uint8_t *dst, size_t dst_len, size
     FIELD_GUARD(f);
     const char *name = "mul_add_matrix_scattered";
     int rc = 0;
-    if (!check_scattered_rows(f, name, dst_len, row_len, row_starts, nrows, &rc)) return rc;
-    for (size_t t = 0; t < nterms; ++t) {
-        if (term_ncoeffs[t] != nrows)
-            return fail("%s: term supplies %zu coefficients for %zu rows", name, term_ncoeffs[t], nrows);
-        if (term_src_lens[t] != row_len)
-            return fail("%s: source is %zu bytes, expected %zu", name, term_src_lens[t], row_len);
-    }
-    if (!rows_apart(name, dst, row_starts, nrows, row_len, term_srcs, term_src_lens, nterms, &rc))
+    if (!check_scattered_terms(f, name, dst, dst_len, row_len, row_starts, nrows, term_ncoeffs, term_srcs,
+                               term_src_lens, nterms, &rc))
         return rc;
     if (nrows == 0 || nterms == 0 || row_len == 0) return FGF_OK;
     if (!terms_present(term_coeffs, nterms, &rc)) return rc;
     CallTables ct;
-    if ((rc = call_tables(f, "mul_add_matrix_scattered", nterms * nrows, nrows, 1, TermCoeffs{term_coeffs, nrows, field_bytes(f)},
+    if ((rc = call_tables(f, name, nterms * nrows, nrows, 1, TermCoeffs{term_coeffs, nrows, field_bytes(f)},
                           S(stream), ct)))
         return rc;
     return run_matrix(f, ct.tb, rows_at(dst, row_starts, nrows), term_srcs, nterms, row_len, false,

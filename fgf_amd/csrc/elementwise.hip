@@ -38,60 +38,15 @@ using namespace core;
 // Compile-time field arithmetic for the DELTA / alpha constants.
 // ---------------------------------------------------------------------------
 
-constexpr uint64_t delta_of(unsigned bits) {
-    return bits == 16 ? 0x20u : bits == 32 ? 0x2000u : 0x20000000ull;
-}
-
-// GF(2^2^n) tower product over Gf8B, as HostField::gmul.
-constexpr uint64_t gf_mul_ce(uint64_t x, uint64_t y, unsigned bits) {
-    if (bits == 8) return poly_mul8((uint8_t)x, (uint8_t)y, 0x1B);
-    const unsigned h = bits / 2;
-    const uint64_t m = lowbits(h);
-    const uint64_t x0 = x & m, x1 = x >> h, y0 = y & m, y1 = y >> h;
-    const uint64_t p00 = gf_mul_ce(x0, y0, h), p11 = gf_mul_ce(x1, y1, h);
-    const uint64_t lo = p00 ^ gf_mul_ce(delta_of(bits), p11, h);
-    const uint64_t hi = gf_mul_ce(x0 ^ x1, y0 ^ y1, h) ^ p00;
-    return lo | (hi << h);
-}
-
-constexpr uint8_t byte_mul_ce(bool fp, uint8_t a, uint8_t b) {
-    return fp ? (uint8_t)fp_mul_bits(a, b, 8) : poly_mul8(a, b, 0x1B);
-}
-
-// The Karatsuba byte constants of a constant multiply, in the branch order
-// of HostField::kara_constants.
+// The Karatsuba byte constants of a constant multiply (kara_constants_of).
 struct KaraConsts {
     uint8_t c[27];
 };
 
-constexpr void kara_ce(bool fp, uint64_t c, unsigned bits, uint8_t *out) {
-    if (bits == 8) {
-        out[0] = (uint8_t)c;
-        return;
-    }
-    const unsigned h = bits / 2;
-    unsigned eh = 1;
-    for (unsigned w = 8; w < h; w *= 2) eh *= 3;
-    const uint64_t c0 = c & lowbits(h), c1 = c >> h;
-    uint64_t s = 0, o1 = 0, o2 = 0;
-    if (fp) {
-        s = c1;
-        o1 = c0 ^ c1;
-        o2 = c0 ^ c1 ^ fp_alpha(c1, h);
-    } else {
-        s = c0;
-        o1 = gf_mul_ce(delta_of(bits), c1, h);
-        o2 = c0 ^ c1;
-    }
-    kara_ce(fp, s, h, out);
-    kara_ce(fp, o1, h, out + eh);
-    kara_ce(fp, o2, h, out + 2 * eh);
-}
-
 template <unsigned BITS, bool FP, uint64_t C>
 constexpr KaraConsts kara_consts_of() {
     KaraConsts k{};
-    kara_ce(FP, C, BITS, k.c);
+    kara_constants_of<BITS>(FP, C, k.c);
     return k;
 }
 
@@ -101,15 +56,9 @@ struct ConstTab {
 };
 
 constexpr ConstTab const_tab(bool fp, uint8_t l) {
-    ConstTab t{};
-    for (unsigned i = 0; i < 4; ++i) {
-        t.t0lo |= (uint32_t)byte_mul_ce(fp, l, (uint8_t)i) << (8 * i);
-        t.t0hi |= (uint32_t)byte_mul_ce(fp, l, (uint8_t)(i + 4)) << (8 * i);
-        t.t1lo |= (uint32_t)byte_mul_ce(fp, l, (uint8_t)(i << 3)) << (8 * i);
-        t.t1hi |= (uint32_t)byte_mul_ce(fp, l, (uint8_t)((i + 4) << 3)) << (8 * i);
-        t.t2 |= (uint32_t)byte_mul_ce(fp, l, (uint8_t)(i << 6)) << (8 * i);
-    }
-    return t;
+    uint32_t e[TAB_DW] = {};
+    byte_const_table(fp ? FP8 : GF8B, l, e);
+    return {e[0], e[1], e[2], e[3], e[4]};
 }
 
 // ---------------------------------------------------------------------------
@@ -172,7 +121,7 @@ __device__ __forceinline__ void mul_planes(const uint32_t *x, const uint32_t *y,
         }
         mul_planes<H, FP>(sx, sy, pm, T);
         if constexpr (!FP) {
-            mul_const<H, false, delta_of(8u * B)>(p11, d);
+            mul_const<H, false, tower_delta_of(8u * B)>(p11, d);
 #pragma unroll
             for (int q = 0; q < H; ++q) {
                 out[q] = p00[q] ^ d[q];
@@ -307,16 +256,16 @@ struct DevLog {
 };
 
 template <unsigned BITS>
-__device__ __forceinline__ uint64_t tower_mul(const DevLog &t, uint64_t x, uint64_t y) {
+__device__ __forceinline__ uint64_t tower_mul_log(const DevLog &t, uint64_t x, uint64_t y) {
     if constexpr (BITS == 8) {
         return t.mul((uint32_t)x, (uint32_t)y);
     } else {
         constexpr unsigned H = BITS / 2;
         constexpr uint64_t MASK = (1ull << H) - 1;
         const uint64_t x0 = x & MASK, x1 = x >> H, y0 = y & MASK, y1 = y >> H;
-        const uint64_t p00 = tower_mul<H>(t, x0, y0), p11 = tower_mul<H>(t, x1, y1);
-        const uint64_t lo = p00 ^ tower_mul<H>(t, delta_of(BITS), p11);
-        const uint64_t hi = tower_mul<H>(t, x0 ^ x1, y0 ^ y1) ^ p00;
+        const uint64_t p00 = tower_mul_log<H>(t, x0, y0), p11 = tower_mul_log<H>(t, x1, y1);
+        const uint64_t lo = p00 ^ tower_mul_log<H>(t, tower_delta_of(BITS), p11);
+        const uint64_t hi = tower_mul_log<H>(t, x0 ^ x1, y0 ^ y1) ^ p00;
         return lo | (hi << H);
     }
 }
@@ -340,7 +289,7 @@ __device__ __forceinline__ uint64_t elem_mul(const DevLog &t, uint64_t x, uint64
     constexpr unsigned BITS = 8u * field_bytes(F);
     if constexpr (F == GF8B || F == GF8D || F == FP8) return t.mul((uint32_t)x, (uint32_t)y);
     else if constexpr (is_fan_paar(F)) return fp_mul<BITS>(t, x, y);
-    else return tower_mul<BITS>(t, x, y);
+    else return tower_mul_log<BITS>(t, x, y);
 }
 
 constexpr int GEN_BLOCK = 256;

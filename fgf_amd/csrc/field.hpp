@@ -11,7 +11,7 @@
 // Elements are packed little endian (src/field/mod.rs:9-15): byte q of an
 // element is its coordinate on the q-th K-basis vector, so multiplying by a
 // fixed c is K-linear in the element's bytes and composes from byte-constant
-// multiplies (HostField::kara_constants).
+// multiplies (kara_constants_of).
 #pragma once
 
 #include <cstddef>
@@ -117,6 +117,72 @@ FGF_HD constexpr ByteLogTables make_log_tables(int bf) {
 }
 
 // ---------------------------------------------------------------------------
+// Constant multiplies composed by Karatsuba from byte constants: the same
+// code on the host (plan tables), on the device (per-call table builds) and
+// at compile time (the elementwise kernel's DELTA and alpha constants).
+// ---------------------------------------------------------------------------
+
+FGF_HD constexpr uint64_t tower_delta_of(unsigned bits) {
+    return bits == 16 ? 0x20u : bits == 32 ? 0x2000u : 0x20000000ull;
+}
+
+// Product in byte field bf from the bit-level definitions.
+FGF_HD constexpr uint8_t byte_mul(int bf, uint8_t a, uint8_t b) {
+    return bf == GF8D ? poly_mul8(a, b, 0x1D) : bf == GF8B ? poly_mul8(a, b, 0x1B) : (uint8_t)fp_mul_bits(a, b, 8);
+}
+
+// GF(2^2^n) tower product over Gf8B, u^2 = u + DELTA (gf16.rs:143-153,
+// gf32.rs:121-130, gf64.rs:120-129).
+template <unsigned BITS>
+FGF_HD constexpr uint64_t tower_mul(uint64_t x, uint64_t y) {
+    if constexpr (BITS == 8) {
+        return poly_mul8((uint8_t)x, (uint8_t)y, 0x1B);
+    } else {
+        constexpr unsigned h = BITS / 2;
+        const uint64_t m = lowbits(h);
+        const uint64_t x0 = x & m, x1 = x >> h, y0 = y & m, y1 = y >> h;
+        const uint64_t p00 = tower_mul<h>(x0, y0), p11 = tower_mul<h>(x1, y1);
+        const uint64_t lo = p00 ^ tower_mul<h>(tower_delta_of(BITS), p11);
+        const uint64_t hi = tower_mul<h>(x0 ^ x1, y0 ^ y1) ^ p00;
+        return lo | (hi << h);
+    }
+}
+
+// The byte constants whose products compose multiplication by c (a
+// BITS-wide element of a tower or Fan-Paar field) by Karatsuba, in the branch
+// order (S, O1, O2) at every level, 3^log2(BITS/8) bytes in all. With
+// c = c0 + c1*u over the half field and x = a + b*u:
+//   towers, u^2 = u + D (gf16.rs:143-153): c*x = (c0 a + D c1 b)
+//     + (c1 a + (c0 + c1) b) u, so S = c0 on a, O1 = D c1 on b,
+//     O2 = c0 + c1 on a + b: lo = S + O1, hi = S + O2.
+//   Fan-Paar, X^2 = alpha X + 1 (fan_paar.rs:124-145): c*x = (c0 a + c1 b)
+//     + (c1 a + (c0 + alpha c1) b) X, so S = c1 on a + b, O1 = c0 + c1 on
+//     a, O2 = c0 + c1 + alpha c1 on b: lo = S + O1, hi = S + O2.
+template <unsigned BITS>
+FGF_HD constexpr void kara_constants_of(bool fp, uint64_t c, uint8_t *out) {
+    if constexpr (BITS == 8) {
+        out[0] = (uint8_t)c;
+    } else {
+        constexpr unsigned h = BITS / 2;
+        constexpr unsigned eh = h == 8 ? 1 : h == 16 ? 3 : 9;
+        const uint64_t c0 = c & lowbits(h), c1 = c >> h;
+        uint64_t s = 0, o1 = 0, o2 = 0;
+        if (fp) {
+            s = c1;
+            o1 = c0 ^ c1;
+            o2 = c0 ^ c1 ^ fp_alpha(c1, h);
+        } else {
+            s = c0;
+            o1 = tower_mul<h>(tower_delta_of(BITS), c1);
+            o2 = c0 ^ c1;
+        }
+        kara_constants_of<h>(fp, s, out);
+        kara_constants_of<h>(fp, o1, out + eh);
+        kara_constants_of<h>(fp, o2, out + 2 * eh);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Host arithmetic over any field, via the byte field's log/exp tables.
 // ---------------------------------------------------------------------------
 
@@ -179,40 +245,6 @@ struct HostField {
             e >>= 1;
         }
         return r;
-    }
-
-    // The byte constants whose products compose multiplication by c (a
-    // `bits`-wide element of field f) by Karatsuba, in the branch order
-    // (S, O1, O2) at every level, 3^log2(bits/8) bytes in all. With c = c0 +
-    // c1*u over the half field and x = a + b*u:
-    //   towers, u^2 = u + D (gf16.rs:143-153): c*x = (c0 a + D c1 b)
-    //     + (c1 a + (c0 + c1) b) u, so S = c0 on a, O1 = D c1 on b,
-    //     O2 = c0 + c1 on a + b: lo = S + O1, hi = S + O2.
-    //   Fan-Paar, X^2 = alpha X + 1 (fan_paar.rs:124-145): c*x = (c0 a + c1 b)
-    //     + (c1 a + (c0 + alpha c1) b) X, so S = c1 on a + b, O1 = c0 + c1 on
-    //     a, O2 = c0 + c1 + alpha c1 on b: lo = S + O1, hi = S + O2.
-    static void kara_constants(int f, uint64_t c, unsigned bits, uint8_t *out) {
-        if (bits == 8) {
-            out[0] = (uint8_t)c;
-            return;
-        }
-        const unsigned h = bits / 2;
-        unsigned eh = 1;
-        for (unsigned w = 8; w < h; w *= 2) eh *= 3;
-        const uint64_t c0 = c & lowbits(h), c1 = c >> h;
-        uint64_t s, o1, o2;
-        if (is_fan_paar(f)) {
-            s = c1;
-            o1 = c0 ^ c1;
-            o2 = c0 ^ c1 ^ fp_alpha(c1, h);
-        } else {
-            s = c0;
-            o1 = gmul(tower_delta(bits), c1, h);
-            o2 = c0 ^ c1;
-        }
-        kara_constants(f, s, h, out);
-        kara_constants(f, o1, h, out + eh);
-        kara_constants(f, o2, h, out + 2 * eh);
     }
 };
 

@@ -22,28 +22,8 @@ namespace fgf {
 // Host: table construction.
 // ---------------------------------------------------------------------------
 
-// One v_perm table set per byte constant of the coefficient: the constant
-// itself for byte fields, the Karatsuba constants of HostField::kara_constants
-// for the towers.
-void build_coeff_table(int field, uint64_t c, uint32_t *out) {
-    const int nb = field_bytes(field);
-    const int bf = byte_field(field);
-    uint8_t m[27];
-    HostField::kara_constants(field, c, 8u * (unsigned)nb, m);
-    for (int i = 0; i < kterms(nb); ++i) {
-        const uint8_t l = m[i];
-        uint8_t t0[8], t1[8], t2[4];
-        for (int i = 0; i < 8; ++i) {
-            t0[i] = HostField::bmul(bf, l, (uint8_t)i);
-            t1[i] = HostField::bmul(bf, l, (uint8_t)(i << 3));
-        }
-        for (int i = 0; i < 4; ++i) t2[i] = HostField::bmul(bf, l, (uint8_t)(i << 6));
-        uint32_t *e = out + i * TAB_DW;
-        std::memcpy(&e[0], t0, 8);
-        std::memcpy(&e[2], t1, 8);
-        std::memcpy(&e[4], t2, 4);
-    }
-}
+// One v_perm table set per byte constant of the coefficient (coeff_table).
+void build_coeff_table(int field, uint64_t c, uint32_t *out) { coeff_table(field, c, out); }
 
 namespace {
 

@@ -6,6 +6,8 @@
 #include <cstddef>
 #include <cstdint>
 
+#include "field.hpp"
+
 namespace fgf {
 
 // Dwords per byte-constant table: {T0 lo, T0 hi, T1 lo, T1 hi, T2}, where
@@ -73,6 +75,41 @@ hipError_t launch_elementwise(int field, uint8_t *dst, const uint8_t *a, const u
 
 // dst[0..len) = 0.
 hipError_t launch_fill_zero(uint8_t *dst, size_t len, hipStream_t stream);
+
+// The tables of byte constant l in byte field bf at e[0..TAB_DW). A
+// product by l is GF(2)-linear in the other factor, so the 20 table bytes
+// are XORs of the eight products l * 2^b.
+FGF_HD constexpr void byte_const_table(int bf, uint8_t l, uint32_t *e) {
+    uint8_t p[8] = {};
+    for (int b = 0; b < 8; ++b) p[b] = byte_mul(bf, l, (uint8_t)(1u << b));
+    auto prod = [&](unsigned x) {
+        uint8_t r = 0;
+        for (int b = 0; b < 8; ++b)
+            if (x >> b & 1) r ^= p[b];
+        return (uint32_t)r;
+    };
+    for (int d = 0; d < TAB_DW; ++d) e[d] = 0;
+    for (unsigned i = 0; i < 4; ++i) {
+        e[0] |= prod(i) << (8 * i);
+        e[1] |= prod(i + 4) << (8 * i);
+        e[2] |= prod(i << 3) << (8 * i);
+        e[3] |= prod((i + 4) << 3) << (8 * i);
+        e[4] |= prod(i << 6) << (8 * i);
+    }
+}
+
+// The entry_dwords(B) dwords of coefficient c of field f: one table per
+// Karatsuba byte constant (the constant itself for byte fields).
+FGF_HD constexpr void coeff_table(int f, uint64_t c, uint32_t *out) {
+    uint8_t m[27] = {};
+    const bool fp = is_fan_paar(f);
+    const int nb = field_bytes(f);
+    if (nb == 1) m[0] = (uint8_t)c;
+    else if (nb == 2) kara_constants_of<16>(fp, c, m);
+    else if (nb == 4) kara_constants_of<32>(fp, c, m);
+    else kara_constants_of<64>(fp, c, m);
+    for (int i = 0; i < kterms(nb); ++i) byte_const_table(byte_field(f), m[i], out + i * TAB_DW);
+}
 
 // Host-side table builder: writes the entry_dwords(B) dwords of coefficient c.
 void build_coeff_table(int field, uint64_t c, uint32_t *out);
