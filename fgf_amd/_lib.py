@@ -73,6 +73,8 @@ PROTOTYPES = {
                                                  c_ptrs, c_sizes, c_size, c_vp]),
     "fgf_mul_add_matrix_scattered": (c_int, [c_int, c_u8p, c_size, c_size, c_sizes, c_size, c_ptrs,
                                              c_sizes, c_ptrs, c_sizes, c_size, c_vp]),
+    "fgf_mul_add_matrix_scattered_noalloc": (c_int, [c_int, c_u8p, c_size, c_size, c_sizes, c_size,
+                                                     c_ptrs, c_sizes, c_ptrs, c_sizes, c_size, c_vp]),
     "fgf_mul_add_matrix_scattered_with": (c_int, [c_int, c_u8p, c_size, c_size, c_sizes, c_size,
                                                   c_plan, c_ptrs, c_sizes, c_size, c_vp]),
     "fgf_dot_product_matrix_scattered_with": (c_int, [c_int, c_u8p, c_size, c_size, c_sizes,

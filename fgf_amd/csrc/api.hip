@@ -1247,6 +1247,103 @@ int fgf_dot_product_matrix_scattered_with(fgf_field f, uint8_t *dst, size_t dst_
                           nrows, plan, srcs, src_lens, nsrcs, true, S(stream));
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free scattered form (no counterpart in the reference): the
+// one-shot mul_add_matrix_scattered for callers that cannot allocate device
+// memory (code under graph capture, tight serving loops). Its per-call
+// coefficient tables go to `scratch`, a fixed array on each device, built
+// there by a kernel that takes the raw coefficients in its arguments.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+constexpr size_t SCRATCH_BYTES = 256u << 10;
+constexpr int SCRATCH_MAX_DEVICES = 64;
+// Coefficients per table-build launch (their values travel in the kernel
+// arguments; one thread builds one coefficient's tables).
+constexpr int BUILD_CHUNK = 256;
+
+// Every device holds its own copy until the process ends. In host code this
+// name is only a handle for hipGetSymbolAddress: kernels get the device
+// address from scratch_ptr().
+static __device__ __align__(256) uint32_t scratch[SCRATCH_BYTES / 4];
+
+std::once_flag g_scratch_once[SCRATCH_MAX_DEVICES];
+uint32_t *g_scratch_ptr[SCRATCH_MAX_DEVICES];
+hipError_t g_scratch_err[SCRATCH_MAX_DEVICES];
+
+// The current device's copy of `scratch`, resolved once per device with that
+// device current.
+int scratch_ptr(uint32_t **out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return device_fail("hipGetDevice", e);
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return fail("scratch: device %d out of range", dev);
+    std::call_once(g_scratch_once[dev], [dev] {
+        g_scratch_err[dev] = hipGetSymbolAddress((void **)&g_scratch_ptr[dev], HIP_SYMBOL(scratch));
+    });
+    if (g_scratch_err[dev] != hipSuccess) return device_fail("hipGetSymbolAddress(scratch)", g_scratch_err[dev]);
+    *out = g_scratch_ptr[dev];
+    return FGF_OK;
+}
+
+struct BuildArgs {
+    uint64_t coeff[BUILD_CHUNK];
+    uint32_t n;
+    int field;
+};
+
+// out[i * entry_dwords(B) ...] = the tables of coefficient i (coeff_table).
+__global__ __launch_bounds__(BUILD_CHUNK) void k_build_tables(uint32_t *out, const BuildArgs a) {
+    const uint32_t i = threadIdx.x;
+    if (i >= a.n) return;
+    coeff_table(a.field, a.coeff[i], out + (size_t)i * entry_dwords(field_bytes(a.field)));
+}
+
+}  // namespace
+
+extern "C" {
+
+int fgf_mul_add_matrix_scattered_noalloc(fgf_field f, uint8_t *dst, size_t dst_len, size_t row_len,
+                                         const size_t *row_starts, size_t nrows,
+                                         const void *const *term_coeffs, const size_t *term_ncoeffs,
+                                         const uint8_t *const *term_srcs, const size_t *term_src_lens,
+                                         size_t nterms, void *stream) {
+    FIELD_GUARD(f);
+    const char *name = "mul_add_matrix_scattered_noalloc";
+    int rc = 0;
+    if (!check_scattered_terms(f, name, dst, dst_len, row_len, row_starts, nrows, term_ncoeffs, term_srcs,
+                               term_src_lens, nterms, &rc))
+        return rc;
+    if (nrows == 0 || nterms == 0 || row_len == 0) return FGF_OK;
+    if (!terms_present(term_coeffs, nterms, &rc)) return rc;
+    const size_t edw = entry_dw(field_bytes(f));
+    size_t n;
+    if (!checked_mul(nterms, nrows, &n) || n > SCRATCH_BYTES / 4 / edw)
+        return fail("%s: %zu x %zu coefficients of %s need more than the %zu-byte scratch (at most %zu "
+                    "coefficients)",
+                    name, nterms, nrows, field_name(f), SCRATCH_BYTES, SCRATCH_BYTES / 4 / edw);
+    uint32_t *sc = nullptr;
+    if ((rc = scratch_ptr(&sc))) return rc;
+    const TermCoeffs c{term_coeffs, nrows, field_bytes(f)};
+    BuildArgs ba;
+    ba.field = f;
+    for (size_t i0 = 0; i0 < n; i0 += BUILD_CHUNK) {
+        ba.n = (uint32_t)std::min<size_t>(BUILD_CHUNK, n - i0);
+        for (uint32_t i = 0; i < ba.n; ++i) ba.coeff[i] = c(i0 + i);
+        hipLaunchKernelGGL(k_build_tables, dim3(1), dim3(BUILD_CHUNK), 0, S(stream), sc + i0 * edw, ba);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return device_fail("table build launch", e);
+    }
+    Tables tb;
+    tb.dev = sc;
+    tb.stride_t = nrows * edw;
+    tb.stride_j = edw;
+    return run_matrix(f, tb, rows_at(dst, row_starts, nrows), term_srcs, nterms, row_len, false, S(stream));
+}
+
 int fgf_mul_elementwise(fgf_field f, uint8_t *dst, size_t dst_len, const uint8_t *a, size_t a_len,
                         const uint8_t *b, size_t b_len, void *stream) {
     FIELD_GUARD(f);

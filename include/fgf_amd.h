@@ -234,6 +234,23 @@ int fgf_mul_add_matrix_scattered(fgf_field field, uint8_t *dst, size_t dst_len, 
                                  const void *const *term_coeffs, const size_t *term_ncoeffs,
                                  const uint8_t *const *term_srcs, const size_t *term_src_lens,
                                  size_t nterms, void *stream);
+/* Allocation-free variant of fgf_mul_add_matrix_scattered. No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under hipGraph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, result and checks (messages name
+ * "mul_add_matrix_scattered_noalloc"). It calls no hipMalloc, hipFree or
+ * other allocating routine: the coefficient tables are built on the device
+ * into a fixed 256 KiB scratch array that each GPU holds until the process
+ * ends, so nterms * nrows is limited to 256 KiB / (4 * entry_dwords):
+ * 13107 coefficients of a byte field, 4369 / 1456 / 485 of a 2- / 4- /
+ * 8-byte field (FGF_EINVAL beyond). Every call on one GPU shares that
+ * GPU's scratch: calls that overlap on one GPU (two host threads, or two
+ * streams not ordered with each other) race and must not be made. */
+int fgf_mul_add_matrix_scattered_noalloc(fgf_field field, uint8_t *dst, size_t dst_len,
+                                         size_t row_len, const size_t *row_starts, size_t nrows,
+                                         const void *const *term_coeffs, const size_t *term_ncoeffs,
+                                         const uint8_t *const *term_srcs, const size_t *term_src_lens,
+                                         size_t nterms, void *stream);
 /* Prepared form of the scattered shape: the reference has none; this is
  * mul_add_matrix_scattered with a (nsrcs, nrows) plan, so decode matrices can
  * stay resident on the device. Same checks and messages as the one-shot form. */
