@@ -53,61 +53,198 @@ def measured_traffic(shard_bytes: int):
     return None
 
 
-def cpu_baseline(seconds_budget: float = 16.0) -> dict:
-    """Time the C port of the reference's x86 Gf8D matrix path on host cores.
+def _time_calls(fn, budget_s: float, warmup: int = 2, reps: int = 1, min_samples: int = 3) -> float:
+    """Median seconds per call over samples of `reps` calls after `warmup`
+    calls, sampling until `budget_s` has passed (at least `min_samples`
+    samples): the method of benches/kernels.rs:41-64 at a bounded budget."""
+    import numpy as np
+    for _ in range(warmup):
+        fn()
+    samples = []
+    t_end = time.perf_counter() + budget_s
+    while len(samples) < 64 and (time.perf_counter() < t_end or len(samples) < min_samples):
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        samples.append((time.perf_counter() - t0) / reps)
+    return float(np.median(samples))
 
-    Sample: RS(10+4) with 16 MiB shards (the headline shape at 1/6.4 of the
-    row length), timed like benches/kernels.rs:41-64: 16 warm-up calls, then
-    samples of 32 calls each, at most 64 samples or half the time budget per
-    core count, and the median sample. First on 1 core (the reference is
-    single-threaded per call), then on all cores. About 12 s of CPU work.
+
+def cpu_baseline() -> dict:
+    """Time the CPU restatement of the reference's dispatched x86 paths
+    (oracle/cpu_port.c, oracle/cpu_port_towers.c, the scalar oracle) on this
+    host's cores, one bounded sample per benchmark configuration, each
+    checked bit-exact against the scalar oracle on a slice before it is
+    timed. About 25 s of CPU work in all.
+
+    Headline value: RS(10+4) Gf8D encode with 16 MiB shards (the headline
+    shape at 1/6.4 of the row length), 16 warm-up calls then samples of 32
+    calls as benches/kernels.rs:41-64, on 1 core (the reference is
+    single-threaded per call) and on all cores with the row split.
     """
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import numpy as np
 
     import oracle_lib as ol
-    from fgf_amd import Gf8D  # noqa: F401
+    from fgf_amd import Gf8D, Gf16, codec, workloads
 
-    L = 16 << 20
+    gfni = ol.has_gfni()
+    backend = 0 if gfni else 1
+    name = "v3_gfni_crypto" if gfni else "v3"
+    ncores = len(os.sched_getaffinity(0))
     rng = np.random.default_rng(1)
+    cfgs = []
+
+    def check(got, want, what):
+        assert np.array_equal(got, want), f"CPU baseline output differs from the oracle: {what}"
+
+    # Config: RS(10+4) encode (the headline's shape, 16 MiB shards).
+    L = 16 << 20
     srcs = [rng.integers(0, 256, L, dtype=np.uint8) for _ in range(K)]
     coeffs = np.array(cauchy_plan_values(K, M), dtype=np.uint8)
     rows = np.zeros(M * L, dtype=np.uint8)
-    backend = 0 if ol.has_gfni() else 1
-    name = "v3_gfni_crypto" if backend == 0 else "v3"
-    results = {}
-    ncores = len(os.sched_getaffinity(0))
-    warmup, reps = 16, 32
-    for threads in (1, ncores):
-        for _ in range(warmup):
-            assert ol.cpu_port_dot_product_matrix_8d(backend, rows, L, M, coeffs, srcs, threads) == 0
-        samples = []
-        t_end = time.perf_counter() + seconds_budget / 2
-        while len(samples) < 64 and (time.perf_counter() < t_end or len(samples) < 3):
-            t0 = time.perf_counter()
-            for _ in range(reps):
-                ol.cpu_port_dot_product_matrix_8d(backend, rows, L, M, coeffs, srcs, threads)
-            samples.append((time.perf_counter() - t0) / reps)
-        med = float(np.median(samples))
-        results[threads] = ((K + M) * L / med / 1e9, K * L / med / 2**30)
-    # bit-exact check of the timed output against the scalar oracle on a slice
+    head = {}
+    for threads, budget in ((1, 5.0), (ncores, 3.0)):
+        t = _time_calls(lambda: ol.cpu_port_dot_product_matrix_8d(backend, rows, L, M, coeffs, srcs, threads),
+                        budget, warmup=16, reps=32)
+        head[threads] = ((K + M) * L / t / 1e9, K * L / t / 2**30)
     want = np.zeros(M * 4096, dtype=np.uint8)
     ol.mul_add_matrix("gf8d", want, 4096, M, [int(c) for c in coeffs], [s[:4096] for s in srcs], True)
-    got = rows.reshape(M, L)[:, :4096].reshape(-1)
-    assert np.array_equal(got, want), "CPU baseline output differs from the oracle"
+    check(rows.reshape(M, L)[:, :4096].reshape(-1), want, "RS(10+4)")
+    cfgs.append({"config": "Gf8D RS(10+4) encode via dot_product_matrix", "sample": "10 x 16 MiB shards",
+                 "GBps": round(head[1][0], 2), "cores": 1, "all_cores_GBps": round(head[ncores][0], 2),
+                 "backend": name, "path": "oracle/cpu_port.c matrix (x86/gf8.rs:2024-2525)"})
+    del srcs, rows
+
+    # Config: k=64 m=16 scattered reconstruct, 4 MiB shards (the config's exact size).
+    k, m, L = 64, 16, 4 << 20
+    srcs = [rng.integers(0, 256, L, dtype=np.uint8) for _ in range(k)]
+    dcoef = rng.integers(1, 256, k * m, dtype=np.uint8)
+    starts = [int(i) * L for i in rng.permutation(m)]
+    dst = np.zeros(m * L, dtype=np.uint8)
+    w = 4096
+    probe = np.zeros(m * w, dtype=np.uint8)
+    ol.cpu_port_mul_add_matrix_scattered_8d(backend, probe, w, [s // L * w for s in starts], dcoef,
+                                            [s[:w] for s in srcs])
+    want = np.zeros(m * w, dtype=np.uint8)
+    ol.mul_add_matrix_scattered("gf8d", want, w, [s // L * w for s in starts], [int(c) for c in dcoef],
+                                [s[:w] for s in srcs])
+    check(probe, want, "scattered")
+    res = {}
+    for threads, budget in ((1, 3.0), (ncores, 1.5)):
+        t = _time_calls(lambda: ol.cpu_port_mul_add_matrix_scattered_8d(backend, dst, L, starts, dcoef, srcs,
+                                                                        threads), budget)
+        res[threads] = (k + 2 * m) * L / t / 1e9
+    cfgs.append({"config": "Gf8D k=64 m=16 mul_add_matrix_scattered reconstruct", "sample": "64 x 4 MiB (full size)",
+                 "GBps": round(res[1], 2), "cores": 1, "all_cores_GBps": round(res[ncores], 2),
+                 "backend": name,
+                 "path": "GFNI scattered row groups (x86/gf8.rs:2155-2199)" if gfni
+                         else "per-element scalar (gf8.rs:620-622 -> scalar.rs:98-109)"})
+    del srcs, dst
+
+    # Config: towers k=32 m=8 dot_product_matrix, 1 MiB per source (1/16 of the config's rows).
+    k, m, L = 32, 8, 1 << 20
+    srcs = [rng.integers(0, 256, L, dtype=np.uint8) for _ in range(k)]
+    for fname in ("gf16", "gf32", "gf64", "fp16", "fp32", "fp64"):
+        nb = ol.BYTES[fname]
+        vals = [int(v) | 1 for v in rng.integers(0, 2**63, k * m, dtype=np.int64)]
+        vals = [v & ((1 << (8 * nb)) - 1) or 1 for v in vals]
+        rows = np.zeros(m * L, dtype=np.uint8)
+        w = 1024
+        got = np.zeros(m * w, dtype=np.uint8)
+        if ol.cpu_port_dot_product_matrix_tower(fname, got, w, m, vals, [s[:w] for s in srcs]) != 0:
+            cfgs.append({"config": f"{fname} k=32 m=8 dot_product_matrix", "skipped": "path unavailable on this CPU"})
+            continue
+        want = np.zeros(m * w, dtype=np.uint8)
+        ol.mul_add_matrix(fname, want, w, m, vals, [s[:w] for s in srcs], True)
+        check(got, want, fname)
+        t = _time_calls(lambda: ol.cpu_port_dot_product_matrix_tower(fname, rows, L, m, vals, srcs, 1), 1.0,
+                        warmup=1)
+        fp = fname.startswith("fp")
+        cfgs.append({"config": f"{fname} k=32 m=8 dot_product_matrix", "sample": "32 x 1 MiB sources",
+                     "GBps": round((k + m) * L / t / 1e9, 2), "cores": 1,
+                     "backend": "v3 (AVX2 nibble tables)" if fp else "v3_gfni_crypto",
+                     "path": ("per-(term,row) nibble AXPY (x86/fan_paar.rs:263-288)" if fp else
+                              "blocked GFNI matrix (x86/gf16.rs:920-1070)" if fname == "gf16" else
+                              "per-(term,row) GFNI AXPY (kernel/gf32.rs:127-133, gf64.rs:121-127)")})
+    del srcs
+
+    # Config: Gf64 mul_elementwise, the reference's scalar Karatsuba per element.
+    n = 1 << 20
+    a = rng.integers(0, 256, 8 * n, dtype=np.uint8)
+    b = rng.integers(0, 256, 8 * n, dtype=np.uint8)
+    t = _time_calls(lambda: ol.mul_elementwise("gf64", a, b), 1.0, warmup=1)
+    cfgs.append({"config": "Gf64 mul_elementwise", "sample": "2^20 pairs (1/64 of the config)",
+                 "GBps": round(3 * 8 * n / t / 1e9, 3), "cores": 1, "backend": "scalar",
+                 "path": "scalar recursive Karatsuba (kernel/gf64.rs:129-131 -> scalar.rs:119-130), "
+                         "the oracle's restatement"})
+
+    # Config: the batched mix, one CPU call per stripe (64 stripes drawn as the mix).
+    specs = workloads.draw_uneven_mix(Gf8D, 64, seed=4096)
+    bufs = []
+    for sp in specs:
+        ss = [rng.integers(0, 256, sp.row_len, dtype=np.uint8) for _ in range(sp.k)]
+        out = np.zeros(len(sp.out_rows) * sp.row_len, dtype=np.uint8)
+        bufs.append((sp, np.array(sp.plan_values, dtype=np.uint8), ss, out))
+
+    def run_mix():
+        for sp, c, ss, out in bufs:
+            ol.cpu_port_dot_product_matrix_8d(backend, out, sp.row_len, len(sp.out_rows), c, ss, 1)
+
+    t = _time_calls(run_mix, 2.0, warmup=1)
+    mix_bytes = sum(sp.algorithmic_bytes for sp in specs)
+    cfgs.append({"config": "batched uneven-stripe mix (Gf8D, 50% encode / 50% reconstruct)",
+                 "sample": f"64 stripes of the mix ({mix_bytes} algorithmic bytes), one dot_product_matrix each",
+                 "GBps": round(mix_bytes / t / 1e9, 2), "cores": 1, "backend": name})
+    del bufs
+
+    # Config: the serving mix, requests one at a time on one core (64 requests).
+    sspecs = workloads.draw_serving_mix([(Gf8D, 0.8), (Gf16, 0.2)], 64, seed=77)
+    reqs = []
+    for sp in sspecs:
+        if sp.erased:
+            D, _ = codec.decode_matrix(sp.field, sp.k, sp.m, sp.erased)
+            vals = codec.plan_values(D)
+        else:
+            vals = codec.encode_plan_values(sp.field, sp.k, sp.m)
+        nrow = len(sp.out_shards)
+        ss = [rng.integers(0, 256, sp.row_len, dtype=np.uint8) for _ in range(sp.k)]
+        reqs.append((sp, vals, ss, np.zeros(nrow * sp.row_len, dtype=np.uint8)))
+    lat = []
+
+    def serve_all():
+        lat.clear()
+        for sp, vals, ss, out in reqs:
+            t0 = time.perf_counter()
+            if sp.field is Gf8D:
+                ol.cpu_port_dot_product_matrix_8d(backend, out, sp.row_len, len(sp.out_shards),
+                                                  np.array(vals, dtype=np.uint8), ss, 1)
+            else:
+                ol.cpu_port_dot_product_matrix_tower("gf16", out, sp.row_len, len(sp.out_shards), vals, ss, 1)
+            lat.append(time.perf_counter() - t0)
+
+    t = _time_calls(serve_all, 2.0, warmup=1)
+    sbytes = sum((sp.k + len(sp.out_shards)) * sp.row_len for sp in sspecs)
+    cfgs.append({"config": "serving mix (80% Gf8D / 20% Gf16 requests)",
+                 "sample": "64 requests served back to back on one core",
+                 "GBps": round(sbytes / t / 1e9, 2), "cores": 1, "backend": name,
+                 "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                 "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3)})
+
     return {
-        "value": round(results[1][0], 2),
+        "value": round(head[1][0], 2),
         "unit": "GB/s",
         "cores": 1,
         "kind": "port",
         "sample": f"RS(10+4) Gf8D encode, 10 x 16 MiB shards, median of samples of 32 calls "
-                  f"after 16 warm-up calls (about 12 s of CPU work); "
-                  f"C port of the reference's {name} path (oracle/cpu_port.c)",
-        "all_cores": {"cores": ncores, "value": round(results[ncores][0], 2), "unit": "GB/s"},
-        "source_GiBps_1core": round(results[1][1], 2),
+                  f"after 16 warm-up calls; C port of the reference's {name} path "
+                  f"(oracle/cpu_port.c)",
+        "all_cores": {"cores": ncores, "value": round(head[ncores][0], 2), "unit": "GB/s"},
+        "source_GiBps_1core": round(head[1][1], 2),
         "backend": name,
-        "gfni": backend == 0,
+        "gfni": gfni,
         "cpu_model": _cpu_model(),
+        "per_config": cfgs,
     }
 
 

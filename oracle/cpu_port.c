@@ -12,6 +12,14 @@
  *  - v3 (AVX2, no GFNI): fill(0) then one split-nibble PSHUFB AXPY per
  *    (term, row) (src/kernel/gf8.rs:536-540, 582-585; nibble tables
  *    src/kernel/tables.rs:79-122).
+ *  - the accumulating scattered matrix: the same 4/2/1 row groups at
+ *    arbitrary disjoint row offsets (src/kernel/x86/gf8.rs:2155-2199) on
+ *    v3_gfni_crypto; v3 dispatches the portable per-element kernel
+ *    (src/kernel/gf8.rs:613-622 -> src/kernel/scalar.rs:98-109).
+ *  - the gather: 128-byte register-blocked tiles seeded from the destination
+ *    (src/kernel/x86/gf8.rs:2811-2897 gather_impl, TILE_LANES 4) on
+ *    v3_gfni_crypto; one nibble AXPY per source on v3 (src/kernel/gf8.rs:
+ *    503-513).
  *
  * Coefficients follow FlatMatrix: coefficient (t, j) is coeffs[t*nrows + j]
  * (src/kernel/mod.rs:235-268).
@@ -61,12 +69,18 @@ static inline __m256i amul(__m256i x, __m256i f) { return _mm256_gf2p8affine_epi
 
 /* gf8.rs:2208-2314 — four rows, 64-byte tiles (2 YMM per row). */
 __attribute__((target("avx2,gfni")))
-static void rows4(uint8_t **p, size_t lo, size_t hi, size_t g, const uint64_t *maps, size_t nrows,
-                  const uint8_t *const *srcs, size_t nterms) {
+static void rows4(uint8_t **p, int acc, size_t lo, size_t hi, size_t g, const uint64_t *maps,
+                  size_t nrows, const uint8_t *const *srcs, size_t nterms) {
     size_t tile = lo;
     for (; tile + 64 <= hi; tile += 64) {
         __m256i a00 = _mm256_setzero_si256(), a01 = a00, a10 = a00, a11 = a00;
         __m256i a20 = a00, a21 = a00, a30 = a00, a31 = a00;
+        if (acc) { /* gf8.rs:2014-2021 load_or_zero */
+            a00 = _mm256_loadu_si256((const __m256i *)(p[0] + tile)); a01 = _mm256_loadu_si256((const __m256i *)(p[0] + tile + 32));
+            a10 = _mm256_loadu_si256((const __m256i *)(p[1] + tile)); a11 = _mm256_loadu_si256((const __m256i *)(p[1] + tile + 32));
+            a20 = _mm256_loadu_si256((const __m256i *)(p[2] + tile)); a21 = _mm256_loadu_si256((const __m256i *)(p[2] + tile + 32));
+            a30 = _mm256_loadu_si256((const __m256i *)(p[3] + tile)); a31 = _mm256_loadu_si256((const __m256i *)(p[3] + tile + 32));
+        }
         for (size_t t = 0; t < nterms; t++) {
             const uint8_t *sp = srcs[t] + tile;
             __m256i x0 = _mm256_loadu_si256((const __m256i *)sp);
@@ -87,13 +101,15 @@ static void rows4(uint8_t **p, size_t lo, size_t hi, size_t g, const uint64_t *m
 
 /* gf8.rs:2321-2480 — two rows / one row, 128-byte tiles (4 YMM per row). */
 __attribute__((target("avx2,gfni")))
-static void rows_wide(uint8_t **p, int nr, size_t lo, size_t hi, size_t g, const uint64_t *maps,
-                      size_t nrows, const uint8_t *const *srcs, size_t nterms) {
+static void rows_wide(uint8_t **p, int nr, int acc, size_t lo, size_t hi, size_t g,
+                      const uint64_t *maps, size_t nrows, const uint8_t *const *srcs, size_t nterms) {
     size_t tile = lo;
     for (; tile + 128 <= hi; tile += 128) {
         __m256i a[2][4];
         for (int r = 0; r < 2; r++)
-            for (int v = 0; v < 4; v++) a[r][v] = _mm256_setzero_si256();
+            for (int v = 0; v < 4; v++)
+                a[r][v] = acc && r < nr ? _mm256_loadu_si256((const __m256i *)(p[r] + tile + 32 * v))
+                                        : _mm256_setzero_si256();
         for (size_t t = 0; t < nterms; t++) {
             const uint8_t *sp = srcs[t] + tile;
             __m256i x[4];
@@ -113,38 +129,37 @@ static size_t tiled_end(size_t lo, size_t hi, size_t tile) { return lo + ((hi - 
 
 /* gf8.rs:2024-2067 — rows grouped 4, then 2, then 1, over columns [lo, hi). */
 __attribute__((target("avx2,gfni")))
-static void matrix_gfni_cols(uint8_t *rows, size_t row_len, size_t nrows, const uint64_t *maps,
+/* gf8.rs:2494-2525: the sub-tile tail, per byte. */
+static void tail_bytes(uint8_t *row, int acc, size_t lo, size_t hi, const uint8_t *coeffs, size_t nrows,
+                       size_t j, const uint8_t *const *srcs, size_t nterms) {
+    for (size_t i = lo; i < hi; i++) {
+        uint8_t v = acc ? row[i] : 0;
+        for (size_t t = 0; t < nterms; t++) v ^= mul8d(coeffs[t * nrows + j], srcs[t][i]);
+        row[i] = v;
+    }
+}
+
+/* gf8.rs:2024-2067 (contiguous rows) and 2155-2199 (scattered rows): rows
+ * grouped 4, then 2, then 1, over columns [lo, hi); row j at rowp[j]. */
+__attribute__((target("avx2,gfni")))
+static void matrix_gfni_cols(uint8_t *const *rowp, int acc, size_t nrows, const uint64_t *maps,
                              const uint8_t *coeffs, const uint8_t *const *srcs, size_t nterms,
                              size_t lo, size_t hi) {
     size_t g = 0;
     while (g + 4 <= nrows) {
-        uint8_t *p[4] = {rows + g * row_len, rows + (g + 1) * row_len, rows + (g + 2) * row_len, rows + (g + 3) * row_len};
+        uint8_t *p[4] = {rowp[g], rowp[g + 1], rowp[g + 2], rowp[g + 3]};
         size_t e = tiled_end(lo, hi, 64);
-        rows4(p, lo, e, g, maps, nrows, srcs, nterms);
-        if (e < hi) {
-            uint8_t *q[4] = {p[0], p[1], p[2], p[3]};
-            /* gf8.rs:2494-2525: the sub-tile tail, per byte. */
-            for (int r = 0; r < 4; r++)
-                for (size_t i = e; i < hi; i++) {
-                    uint8_t acc = 0;
-                    for (size_t t = 0; t < nterms; t++) acc ^= mul8d(coeffs[t * nrows + g + r], srcs[t][i]);
-                    q[r][i] = acc;
-                }
-        }
+        rows4(p, acc, lo, e, g, maps, nrows, srcs, nterms);
+        for (int r = 0; r < 4; r++) tail_bytes(p[r], acc, e, hi, coeffs, nrows, g + r, srcs, nterms);
         g += 4;
     }
     int rest = (int)(nrows - g);
     while (rest > 0) {
         int nr = rest >= 2 ? 2 : 1;
-        uint8_t *p[2] = {rows + g * row_len, rows + (g + (nr > 1 ? 1 : 0)) * row_len};
+        uint8_t *p[2] = {rowp[g], rowp[g + (nr > 1 ? 1 : 0)]};
         size_t e = tiled_end(lo, hi, 128);
-        rows_wide(p, nr, lo, e, g, maps, nrows, srcs, nterms);
-        for (int r = 0; r < nr; r++)
-            for (size_t i = e; i < hi; i++) {
-                uint8_t acc = 0;
-                for (size_t t = 0; t < nterms; t++) acc ^= mul8d(coeffs[t * nrows + g + r], srcs[t][i]);
-                p[r][i] = acc;
-            }
+        rows_wide(p, nr, acc, lo, e, g, maps, nrows, srcs, nterms);
+        for (int r = 0; r < nr; r++) tail_bytes(p[r], acc, e, hi, coeffs, nrows, g + r, srcs, nterms);
         g += (size_t)nr;
         rest -= nr;
     }
@@ -169,20 +184,20 @@ static void axpy_nibble(uint8_t *dst, const nibble_t *tb, const uint8_t *src, si
     for (; i < hi; i++) dst[i] ^= (uint8_t)(tb->lo[src[i] & 15] ^ tb->hi[src[i] >> 4]);
 }
 
-static void matrix_v3_cols(uint8_t *rows, size_t row_len, size_t nrows, const nibble_t *tabs,
+static void matrix_v3_cols(uint8_t *const *rowp, int acc, size_t nrows, const nibble_t *tabs,
                            const uint8_t *const *srcs, size_t nterms, size_t lo, size_t hi) {
-    for (size_t j = 0; j < nrows; j++) memset(rows + j * row_len + lo, 0, hi - lo);
+    if (!acc)
+        for (size_t j = 0; j < nrows; j++) memset(rowp[j] + lo, 0, hi - lo);
     for (size_t t = 0; t < nterms; t++)
-        for (size_t j = 0; j < nrows; j++)
-            axpy_nibble(rows + j * row_len, &tabs[t * nrows + j], srcs[t], lo, hi);
+        for (size_t j = 0; j < nrows; j++) axpy_nibble(rowp[j], &tabs[t * nrows + j], srcs[t], lo, hi);
 }
 
 /* ------------------------------ dispatcher ------------------------------ */
 
 typedef struct {
-    int gfni;
-    uint8_t *rows;
-    size_t row_len, nrows, nterms, lo, hi;
+    int gfni, acc;
+    uint8_t *const *rowp;
+    size_t nrows, nterms, lo, hi;
     const uint8_t *coeffs;
     const uint8_t *const *srcs;
     const uint64_t *maps;
@@ -191,23 +206,17 @@ typedef struct {
 
 static void run_job(const job_t *j) {
     if (j->gfni)
-        matrix_gfni_cols(j->rows, j->row_len, j->nrows, j->maps, j->coeffs, j->srcs, j->nterms, j->lo, j->hi);
+        matrix_gfni_cols(j->rowp, j->acc, j->nrows, j->maps, j->coeffs, j->srcs, j->nterms, j->lo, j->hi);
     else
-        matrix_v3_cols(j->rows, j->row_len, j->nrows, j->tabs, j->srcs, j->nterms, j->lo, j->hi);
+        matrix_v3_cols(j->rowp, j->acc, j->nrows, j->tabs, j->srcs, j->nterms, j->lo, j->hi);
 }
 
 static void *job_thread(void *arg) { run_job((const job_t *)arg); return NULL; }
 
-/*
- * Gf8D dot_product_matrix on the host: rows[j] = sum_t coeffs[t*nrows+j] * srcs[t].
- * backend: 0 = v3_gfni_crypto, 1 = v3 (AVX2 nibble). threads >= 1; with more
- * than one thread row_len is split into 4 KiB-aligned column slices (the
- * reference itself is single-threaded). Returns -1 if the backend is not
- * available on this CPU.
- */
-int cpu_port_dot_product_matrix_8d(int backend, uint8_t *rows, size_t row_len, size_t nrows,
-                                   const uint8_t *coeffs, const uint8_t *const *srcs,
-                                   size_t nterms, int threads) {
+/* Runs the Gf8D matrix over rows rowp[0..nrows) (accumulating or not),
+ * splitting columns over `threads` 4 KiB-aligned slices. */
+static int run_matrix_8d(int backend, uint8_t *const *rowp, int acc, size_t row_len, size_t nrows,
+                         const uint8_t *coeffs, const uint8_t *const *srcs, size_t nterms, int threads) {
     if (backend == 0 && !cpu_port_has_gfni()) return -1;
     if (!__builtin_cpu_supports("avx2")) return -1;
     size_t ncoef = nterms * nrows;
@@ -218,7 +227,7 @@ int cpu_port_dot_product_matrix_8d(int backend, uint8_t *rows, size_t row_len, s
         nibble_8d(coeffs[i], &tabs[i]);
     }
     if (threads < 1) threads = 1;
-    job_t base = {backend == 0, rows, row_len, nrows, nterms, 0, row_len, coeffs, srcs, maps, tabs};
+    job_t base = {backend == 0, acc, rowp, nrows, nterms, 0, row_len, coeffs, srcs, maps, tabs};
     if (threads == 1 || row_len < 8192) {
         run_job(&base);
     } else {
@@ -237,4 +246,57 @@ int cpu_port_dot_product_matrix_8d(int backend, uint8_t *rows, size_t row_len, s
     free(maps);
     free(tabs);
     return 0;
+}
+
+/*
+ * Gf8D dot_product_matrix on the host: rows[j] = sum_t coeffs[t*nrows+j] * srcs[t].
+ * backend: 0 = v3_gfni_crypto, 1 = v3 (AVX2 nibble). threads >= 1; with more
+ * than one thread row_len is split into 4 KiB-aligned column slices (the
+ * reference itself is single-threaded). Returns -1 if the backend is not
+ * available on this CPU.
+ */
+int cpu_port_dot_product_matrix_8d(int backend, uint8_t *rows, size_t row_len, size_t nrows,
+                                   const uint8_t *coeffs, const uint8_t *const *srcs,
+                                   size_t nterms, int threads) {
+    uint8_t **rowp = (uint8_t **)malloc((nrows ? nrows : 1) * sizeof(uint8_t *));
+    for (size_t j = 0; j < nrows; j++) rowp[j] = rows + j * row_len;
+    int rc = run_matrix_8d(backend, rowp, 0, row_len, nrows, coeffs, srcs, nterms, threads);
+    free(rowp);
+    return rc;
+}
+
+/*
+ * Gf8D mul_add_matrix_scattered on the host: row j at dst + row_starts[j]
+ * ^= sum_t coeffs[t*nrows+j] * srcs[t]. backend 0: the GFNI scattered
+ * blocks (gf8.rs:2155-2199); backend 1: what v3 dispatches, the portable
+ * per-element kernel (scalar.rs:98-109, restated by the oracle).
+ */
+int cpu_port_mul_add_matrix_scattered_8d(int backend, uint8_t *dst, size_t row_len,
+                                         const size_t *row_starts, size_t nrows,
+                                         const uint8_t *coeffs, const uint8_t *const *srcs,
+                                         size_t nterms, int threads) {
+    if (backend == 1) {
+        const size_t ncoef = nterms * nrows;
+        uint64_t *c = (uint64_t *)malloc((ncoef ? ncoef : 1) * sizeof(uint64_t));
+        for (size_t i = 0; i < ncoef; i++) c[i] = coeffs[i];
+        oracle_mul_add_matrix_scattered(OR_GF8D, dst, row_len, row_starts, nrows, c, srcs, nterms);
+        free(c);
+        return 0;
+    }
+    uint8_t **rowp = (uint8_t **)malloc((nrows ? nrows : 1) * sizeof(uint8_t *));
+    for (size_t j = 0; j < nrows; j++) rowp[j] = dst + row_starts[j];
+    int rc = run_matrix_8d(backend, rowp, 1, row_len, nrows, coeffs, srcs, nterms, threads);
+    free(rowp);
+    return rc;
+}
+
+/*
+ * Gf8D mul_add_gather on the host: dst ^= sum_t coeffs[t] * srcs[t].
+ * backend 0: 128-byte tiles seeded from dst (gf8.rs:2811-2897); backend 1:
+ * one nibble AXPY per source (gf8.rs:503-513).
+ */
+int cpu_port_mul_add_gather_8d(int backend, uint8_t *dst, size_t len, const uint8_t *coeffs,
+                               const uint8_t *const *srcs, size_t nterms, int threads) {
+    uint8_t *rowp[1] = {dst};
+    return run_matrix_8d(backend, rowp, 1, len, 1, coeffs, srcs, nterms, threads);
 }

@@ -63,6 +63,17 @@ def lib() -> ctypes.CDLL:
         l.cpu_port_dot_product_matrix_8d.argtypes = [
             ctypes.c_int, u8p, ctypes.c_size_t, ctypes.c_size_t, u8p, ctypes.POINTER(u8p),
             ctypes.c_size_t, ctypes.c_int]
+        l.cpu_port_mul_add_matrix_scattered_8d.restype = ctypes.c_int
+        l.cpu_port_mul_add_matrix_scattered_8d.argtypes = [
+            ctypes.c_int, u8p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), ctypes.c_size_t, u8p,
+            ctypes.POINTER(u8p), ctypes.c_size_t, ctypes.c_int]
+        l.cpu_port_mul_add_gather_8d.restype = ctypes.c_int
+        l.cpu_port_mul_add_gather_8d.argtypes = [
+            ctypes.c_int, u8p, ctypes.c_size_t, u8p, ctypes.POINTER(u8p), ctypes.c_size_t, ctypes.c_int]
+        l.cpu_port_dot_product_matrix_tower.restype = ctypes.c_int
+        l.cpu_port_dot_product_matrix_tower.argtypes = [
+            ctypes.c_int, u8p, ctypes.c_size_t, ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint64),
+            ctypes.POINTER(u8p), ctypes.c_size_t, ctypes.c_int]
         _lib = l
     return _lib
 
@@ -152,6 +163,29 @@ def cpu_port_dot_product_matrix_8d(backend: int, rows: np.ndarray, row_len: int,
     c = np.ascontiguousarray(coeffs, dtype=np.uint8)
     return lib().cpu_port_dot_product_matrix_8d(backend, _p(rows), row_len, nrows, _p(c),
                                                 _src_array(srcs), len(srcs), threads)
+
+
+def cpu_port_mul_add_matrix_scattered_8d(backend: int, dst: np.ndarray, row_len: int, row_starts,
+                                         coeffs: np.ndarray, srcs, threads: int = 1) -> int:
+    c = np.ascontiguousarray(coeffs, dtype=np.uint8)
+    rs = (ctypes.c_size_t * max(len(row_starts), 1))(*row_starts)
+    return lib().cpu_port_mul_add_matrix_scattered_8d(backend, _p(dst), row_len, rs, len(row_starts),
+                                                      _p(c), _src_array(srcs), len(srcs), threads)
+
+
+def cpu_port_mul_add_gather_8d(backend: int, dst: np.ndarray, coeffs: np.ndarray, srcs,
+                               threads: int = 1) -> int:
+    c = np.ascontiguousarray(coeffs, dtype=np.uint8)
+    return lib().cpu_port_mul_add_gather_8d(backend, _p(dst), dst.size, _p(c), _src_array(srcs),
+                                            len(srcs), threads)
+
+
+def cpu_port_dot_product_matrix_tower(field: str, rows: np.ndarray, row_len: int, nrows: int,
+                                      coeffs, srcs, threads: int = 1) -> int:
+    c = np.asarray(list(coeffs) or [0], dtype=np.uint64)
+    return lib().cpu_port_dot_product_matrix_tower(FIELDS[field], _p(rows), row_len, nrows,
+                                                   c.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                                   _src_array(srcs), len(srcs), threads)
 
 
 def has_gfni() -> bool:

@@ -188,6 +188,22 @@ def test_planar_transposes_are_inverse_permutations():
 # Packing (tests/ops.rs:813-832).
 # ---------------------------------------------------------------------------
 
+def test_pack_writes_the_reference_layout_vectors():
+    # gf16.rs:33-35: from_components(0x12, 0x34) packs as [0x12, 0x34].
+    import json
+    golden = json.load(open(os.path.join(ROOT, "tests", "golden", "kats.json")))
+    for v in golden["layout"]:
+        f = BY_NAME[v["field"]]
+        raw = int(v["raw"], 16)
+        out = ops.pack_to_vec(f, [raw])
+        assert bytes(out) == raw.to_bytes(f.BYTES, "little")
+        if "bytes" in v:
+            assert list(bytes(out)) == v["bytes"]
+        back = [0]
+        ops.unpack(f, back, bytes(out))
+        assert back == [raw]
+
+
 def test_pack_unpack_round_trip():
     elems = [0, 1, 0x0108, 0xFFFF]
     b = ops.pack_to_vec(Gf16, elems)

@@ -154,3 +154,105 @@ def test_a_row_longer_than_one_launch():
             assert np.array_equal(host(dst[p:p + 4096]), b), (off, p)
     del src, dst
     torch.cuda.empty_cache()
+
+
+def _oracle_sample(field, name, pool_rows, srcs, plan_values, k, m, L, width=4096):
+    """Oracle check of the first and last `width` bytes of m output rows
+    against their k sources under a (k, m) plan."""
+    w = min(width, L)
+    for off in sorted({0, L - w}):
+        hs = [s[off:off + w].cpu().numpy() for s in srcs]
+        want = np.zeros(m * w, np.uint8)
+        ol.mul_add_matrix(name, want, w, m, plan_values, hs, True)
+        got = torch.cat([r[off:off + w] for r in pool_rows]).cpu().numpy()
+        assert np.array_equal(got, want), (name, k, m, L, off)
+
+
+def test_batched_mix_4096_stripes_full_size():
+    # The benchmark's batched workload as bench.py builds it: 4096 Gf8D
+    # stripes, k ~ U[4,32], m ~ U[2,8], L log-uniform 4 KiB-1 MiB, half of
+    # them rebuilding erased shards, about 95k work items in one launch. The
+    # launch and its graph replays equal the per-call kernel byte for byte,
+    # every rebuild returns the erased bytes, and sampled stripes equal the
+    # oracle.
+    from fgf_amd import workloads
+    from fgf_amd.batch import Batch, Stripe
+    specs = workloads.draw_uneven_mix(Gf8D, 4096, seed=4096)
+    pool, saved = workloads.materialize(Gf8D, specs, seed=4097)
+    plans, stripes = {}, []
+    for sp in specs:
+        key = (sp.k, len(sp.out_rows), tuple(sp.plan_values))
+        if key not in plans:
+            plans[key] = ops.Plan.matrix(Gf8D, sp.k, len(sp.out_rows), sp.plan_values)
+        srcs, rows = workloads.stripe_buffers(pool, sp)
+        stripes.append(Stripe(plans[key], srcs, rows, sp.row_len, True))
+    batch = Batch(Gf8D, stripes)
+    assert batch.items > 90000
+
+    def outputs():
+        return [torch.cat(workloads.stripe_buffers(pool, sp)[1]).clone() for sp in specs]
+
+    def clobber():
+        for sp in specs:
+            for r in workloads.stripe_buffers(pool, sp)[1]:
+                r.fill_(0xA5)
+
+    batch.run_per_call()
+    torch.cuda.synchronize()
+    ref = outputs()
+    for runner in (batch.run, batch.run_graph, batch.run_graph):
+        clobber()
+        runner()
+        torch.cuda.synchronize()
+        for i, (sp, got) in enumerate(zip(specs, outputs())):
+            assert torch.equal(got, ref[i]), (runner.__name__, i, sp.kind)
+            if sp.kind == "reconstruct":
+                assert torch.equal(got, saved[i]), (runner.__name__, i)
+    rng = np.random.default_rng(3)
+    for i in rng.choice(len(specs), 12, replace=False):
+        sp = specs[int(i)]
+        srcs, rows = workloads.stripe_buffers(pool, sp)
+        _oracle_sample(Gf8D, "gf8d", rows, srcs, sp.plan_values, sp.k, len(sp.out_rows), sp.row_len)
+    del pool, ref
+    torch.cuda.empty_cache()
+
+
+def test_serving_4096_requests_256_matrices_default_budget():
+    # The benchmark's serving workload: 4096 requests, 80% Gf8D / 20%
+    # Gf16 over 256 distinct coefficient matrices, shard lengths as the
+    # batched mix, through a server with the default configuration (64 GiB
+    # in-flight budget, 256-entry plan cache, 4 slots), once as a burst and
+    # once at Poisson arrivals of 50k req/s. Every request's output equals
+    # its per-call result; sampled encodes equal the oracle.
+    from fgf_amd import workloads
+    from fgf_amd.serve import Server
+    specs = workloads.draw_serving_mix([(Gf8D, 0.8), (Gf16, 0.2)], 4096, seed=77)
+    assert len({(sp.field.id, sp.k, sp.m, tuple(sp.erased)) for sp in specs}) == 256
+    pool, expected = workloads.materialize_serving(specs, seed=78)
+    reqs = [workloads.serve_request(pool, sp) for sp in specs]
+    srv = Server()
+    assert srv.config["hbm_budget_bytes"] == 64 << 30
+    rng = np.random.default_rng(79)
+    for arrivals in ([0.0] * len(reqs), np.cumsum(rng.exponential(1 / 50000.0, len(reqs))).tolist()):
+        for sp in specs:
+            for i in sp.out_shards:
+                workloads.serve_shard(pool, sp, i).zero_()
+        torch.cuda.synchronize()
+        lat, wall = srv.replay(reqs, arrivals)
+        assert all(x >= 0 for x in lat)
+        for sp, want in zip(specs, expected):
+            got = torch.cat([workloads.serve_shard(pool, sp, i) for i in sp.out_shards])
+            assert torch.equal(got, want), sp
+    st = srv.stats()
+    assert st["completed"] == 2 * len(reqs) and st["failed"] == 0
+    srv.close()
+    names = {Gf8D.id: "gf8d", Gf16.id: "gf16"}
+    encodes = [sp for sp in specs if not sp.erased]
+    for j in rng.choice(len(encodes), 8, replace=False):
+        sp = encodes[int(j)]
+        shard = lambda i, sp=sp: workloads.serve_shard(pool, sp, i)
+        _oracle_sample(sp.field, names[sp.field.id], [shard(sp.k + r) for r in range(sp.m)],
+                       [shard(t) for t in range(sp.k)],
+                       codec.encode_plan_values(sp.field, sp.k, sp.m), sp.k, sp.m, sp.row_len)
+    del pool, expected
+    torch.cuda.empty_cache()

@@ -126,6 +126,25 @@ def test_tower_layout_and_defining_relations():
             assert ol.mul("gf32", a, b) == ol.mul("gf8b", a, b)
 
 
+def test_tower_component_layout_vectors():
+    # from_components(lo, hi) of gf16.rs:33-35, gf32.rs:17-18, gf64.rs:17-18:
+    # the element lo + hi*u, u the tower's generator of the upper half. In
+    # the field arithmetic, hi*u (hi taken from the half field) must land in
+    # the upper half, so lo + hi*u is the listed raw value; its packed
+    # little-endian bytes are the listed bytes. A swapped half or a wrong u
+    # fails the product.
+    for v in GOLDEN["layout"]:
+        f = v["field"]
+        bits = 8 * ol.BYTES[f]
+        lo, hi = (int(c, 16) for c in v["components"])
+        u = 1 << (bits // 2)
+        raw = int(v["raw"], 16)
+        assert lo ^ ol.mul(f, hi, u) == raw, f
+        assert ol.mul(f, hi, u) == hi << (bits // 2), f
+        if "bytes" in v:
+            assert list(raw.to_bytes(bits // 8, "little")) == v["bytes"], f
+
+
 def test_fan_paar_nesting_and_mul_alpha():
     # tests/algebra.rs:554-577 nested subfields; fan_paar.rs:52-53 mul_alpha.
     for a, b in ((0x1B, 0xA8), (0x53, 0xCA), (0xFF, 0x42)):
@@ -232,3 +251,46 @@ def test_cpu_baseline_port_is_bit_exact(backend):
         got = ol.noise(row_len * nrows, 1)
         assert ol.cpu_port_dot_product_matrix_8d(backend, got, row_len, nrows, coeffs, srcs, threads) == 0
         assert np.array_equal(got, want), (row_len, nrows, nterms)
+
+
+@pytest.mark.skipif(not ol.has_gfni(), reason="host CPU lacks AVX2+GFNI")
+@pytest.mark.parametrize("backend", [0, 1])
+def test_cpu_baseline_scattered_and_gather_are_bit_exact(backend):
+    # The timed baselines of the scattered reconstruct (GFNI blocks at
+    # scattered offsets, or v3's per-element kernel) and of the gather.
+    for row_len, nrows, nterms, threads in ((1, 1, 1, 1), (63, 3, 2, 1), (200, 5, 10, 1),
+                                            (129, 7, 5, 1), (1 << 15, 16, 12, 4), (9000, 6, 3, 3)):
+        srcs = [ol.noise(row_len, 0xA00 + t) for t in range(nterms)]
+        coeffs = ol.noise(nrows * nterms, 0x78)
+        gap = 40
+        starts = [int(i) * (row_len + gap) for i in np.random.default_rng(nrows).permutation(nrows)]
+        init = ol.noise(nrows * (row_len + gap), 0x44)
+        want = init.copy()
+        ol.mul_add_matrix_scattered("gf8d", want, row_len, starts, [int(c) for c in coeffs], srcs)
+        got = init.copy()
+        assert ol.cpu_port_mul_add_matrix_scattered_8d(backend, got, row_len, starts, coeffs, srcs,
+                                                       threads) == 0
+        assert np.array_equal(got, want), (row_len, nrows, nterms)
+        dst = ol.noise(row_len, 0x45)
+        want1 = dst.copy()
+        for t in range(nterms):
+            ol.mul_add("gf8d", want1, int(coeffs[t]), srcs[t])
+        assert ol.cpu_port_mul_add_gather_8d(backend, dst, coeffs[:nterms], srcs, threads) == 0
+        assert np.array_equal(dst, want1), (row_len, nterms)
+
+
+@pytest.mark.skipif(not ol.has_gfni(), reason="host CPU lacks AVX2+GFNI")
+@pytest.mark.parametrize("field", ["gf16", "gf32", "gf64", "fp16", "fp32", "fp64"])
+def test_cpu_baseline_towers_are_bit_exact(field):
+    nb = ol.BYTES[field]
+    for row_len, nrows, nterms, threads in ((nb, 1, 1, 1), (8 * nb + 24, 3, 2, 1),
+                                            (64 * nb, 5, 18, 1), (40000, 8, 4, 3)):
+        row_len -= row_len % nb
+        srcs = [ol.noise(row_len, 0xB00 + t) for t in range(nterms)]
+        raw = ol.noise(nrows * nterms * nb, 0x79)
+        coeffs = [int.from_bytes(raw[i * nb:(i + 1) * nb].tobytes(), "little") for i in range(nrows * nterms)]
+        want = np.zeros(row_len * nrows, dtype=np.uint8)
+        ol.mul_add_matrix(field, want, row_len, nrows, coeffs, srcs, True)
+        got = ol.noise(row_len * nrows, 2)
+        assert ol.cpu_port_dot_product_matrix_tower(field, got, row_len, nrows, coeffs, srcs, threads) == 0
+        assert np.array_equal(got, want), (field, row_len, nrows, nterms)
