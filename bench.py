@@ -40,13 +40,14 @@ def cauchy_plan_values(k: int, m: int) -> list[int]:
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_matrix_rs10_4_encode_traffic.txt")
 
 
-def measured_traffic(shard_bytes: int):
+def measured_traffic(shard_bytes: int, kernel: str):
     """HBM bytes per encode launch from the committed FETCH_SIZE/WRITE_SIZE
     profile of this workload (counters scaled by a plain-copy calibration);
-    None when the profile does not cover this shard length."""
+    None when the profile does not cover this shard length and kernel."""
     try:
-        kv = dict(line.split(":", 1) for line in open(TRAFFIC_PROFILE) if ":" in line)
-        if int(kv["shard_bytes"]) == shard_bytes:
+        kv = {a.strip(): b.strip() for a, b in (line.split(":", 1) for line in open(TRAFFIC_PROFILE)
+                                                if ":" in line)}
+        if int(kv["shard_bytes"]) == shard_bytes and kv.get("kernel") == kernel:
             return int(kv["traffic_bytes_per_launch"])
     except (OSError, KeyError, ValueError):
         pass
@@ -425,6 +426,11 @@ def _leg(name: str, ms: float, nbytes: int, **extra) -> dict:
             "GBps": round(gbps, 1), "hbm_frac": round(gbps / HBM_PEAK_GBS, 4), **extra}
 
 
+def _kernel_name(plan) -> str:
+    form, rows = plan.kernel()
+    return "packed product tables" if form == 1 else f"v_perm tables, rows per block {rows or 'max'}"
+
+
 def bench_configs(seed: int = 31) -> dict:
     """The remaining workload shapes, each checked against the oracle on a
     slice before timing (median of 20 launches):
@@ -443,7 +449,7 @@ def bench_configs(seed: int = 31) -> dict:
     import numpy as np
     import torch
 
-    from fgf_amd import (FanPaar16, FanPaar32, FanPaar64, Gf8D, Gf16, Gf32, Gf64, codec,
+    from fgf_amd import (FanPaar16, FanPaar32, FanPaar64, Gf8D, Gf16, Gf32, Gf64, autotune, codec,
                          ops)
 
     dev = torch.device("cuda:0")
@@ -495,8 +501,17 @@ def bench_configs(seed: int = 31) -> dict:
                                                     [shard(i) for i in surv])
     run()
     assert torch.equal(torch.cat([shard(e) for e in erased]), saved), "reconstruct mismatch"
+    t_vperm = _median_launch_ms(run)
+    # The per-call autotuner picks the plan's kernel (v_perm tables at
+    # several row groups, or packed 256-entry product tables) on this shape.
+    tuned = autotune.tune_plan(dplan, L)
+    for e in erased:
+        shard(e).zero_()
+    run()
+    assert torch.equal(torch.cat([shard(e) for e in erased]), saved), "reconstruct mismatch (tuned kernel)"
     legs.append(_leg("Gf8D k=64 m=16 scattered reconstruct, 4 MiB shards", _median_launch_ms(run),
-                     (k + 2 * m) * L, erased=erased))
+                     (k + 2 * m) * L, erased=erased, kernel=_kernel_name(dplan),
+                     v_perm_median_ms=round(t_vperm, 4), autotune=tuned))
     del region, saved
 
     # RS(10+4) in the reed-solomon-erasure / klauspost layout over 64 MiB
@@ -528,13 +543,15 @@ def bench_configs(seed: int = 31) -> dict:
     for f in (Gf16, Gf32, Gf64, FanPaar16, FanPaar32, FanPaar64):
         vals = [int(v) for v in rng.integers(1, 1 << f.BITS, k * m, dtype=np.uint64)]
         plan = ops.Plan.matrix(f, k, m, vals)
+        tuned = autotune.tune_plan(plan, L)
         run = lambda: ops.dot_product_matrix_with(f, rows.view(-1), L, m, plan, list(src))
         run()
         want = codec.host_dot_product_matrix(f, vals, [s[:512].cpu().numpy() for s in src], m)
         assert np.array_equal(rows[:, :512].cpu().numpy(), want), f.NAME
         legs.append(_leg(f"{f.NAME} k=32 m=8 dot_product_matrix, 16 MiB sources",
                          _median_launch_ms(run, reps=10), (k + m) * L,
-                         byte_products_per_source_byte=m * f.BYTES))
+                         byte_products_per_source_byte=m * f.BYTES, kernel=_kernel_name(plan),
+                         autotune=tuned))
     del src, rows
 
     # Gf64 elementwise, 2^26 pairs.
@@ -726,6 +743,9 @@ def main() -> int:
     ap.add_argument("--preroll-ms", type=float, default=300.0,
                     help="untimed back-to-back launches for at least this long before the "
                          "counted warm-up (sustained clocks for the timed launches)")
+    ap.add_argument("--kernel", choices=("auto", "vperm", "packed"), default="auto",
+                    help="headline plan kernel: picked by the autotuner (auto), or forced (profiling "
+                         "runs force the kernel the timed run picked)")
     ap.add_argument("--row-len", type=int, default=L_HEADLINE)
     ap.add_argument("--calib-copy", action="store_true",
                     help="after timing, run one 1 GiB device copy (a plain copy kernel of known "
@@ -758,6 +778,17 @@ def main() -> int:
     srcs = [src[t] for t in range(K)]
     plan = ops.Plan.matrix(Gf8D, K, M, cauchy_plan_values(K, M))
     stream = torch.cuda.current_stream(dev)
+    # The plan's per-call kernel (v_perm tables at several row groups, or
+    # packed 256-entry product tables), picked by the autotuner on this shape.
+    from fgf_amd import autotune
+    if args.kernel == "auto":
+        tuned = autotune.tune_plan(plan, L)
+    else:
+        plan.set_kernel(1 if args.kernel == "packed" else 0, 0)
+        tuned = {"choice": list(plan.kernel()), "forced": args.kernel}
+    form, _ = plan.kernel()
+    kernel_name = "k_matrix_ptab<R=4,overwrite>" if form == 1 else "k_matrix<B=1,M=4,D=4,overwrite>"
+    print(f"[bench] autotuned kernel: {kernel_name}", file=sys.stderr, flush=True)
 
     # Spot check of this exact launch (last 64 KiB of every row) against the
     # library's host field arithmetic; bit-exactness against the oracle is
@@ -860,8 +891,9 @@ def main() -> int:
                        "note": "first launches after an idle GPU, before the pre-roll"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(L), "kernel_avg_ms": round(avg_ms, 5),
-                     "kernel": "k_matrix<B=1,M=4,D=4,overwrite>"},
+                     "traffic": measured_traffic(L, kernel_name), "kernel_avg_ms": round(avg_ms, 5),
+                     "kernel": kernel_name},
+        "autotune": tuned,
     }
     rows = None
     if args.calib_copy:

@@ -38,6 +38,8 @@ PROTOTYPES = {
     "fgf_plan_dimensions": (None, [c_plan, c_sizes, c_sizes]),
     "fgf_plan_field": (c_int, [c_plan]),
     "fgf_plan_get": (c_int, [c_plan, c_size, ctypes.POINTER(c_u64)]),
+    "fgf_plan_set_kernel": (c_int, [c_plan, c_int, c_int]),
+    "fgf_plan_get_kernel": (c_int, [c_plan, ctypes.POINTER(c_int), ctypes.POINTER(c_int)]),
     "fgf_plan_get_at": (c_int, [c_plan, c_size, c_size, ctypes.POINTER(c_u64)]),
     "fgf_add_assign": (c_int, [c_int, c_u8p, c_size, c_u8p, c_size, c_vp]),
     "fgf_mul_add": (c_int, [c_int, c_u8p, c_size, c_u64, c_u8p, c_size, c_vp]),

@@ -1,4 +1,4 @@
-"""Autotuner for the batched launch.
+"""Autotuners: the batched launch and the per-call kernel choice of a plan.
 
 Picks the tile and launch shape of `Batch` — dwords per lane (bytes per lane =
 4 * dwords), destination rows per wave, wave tiles per work item, resident
@@ -6,15 +6,22 @@ waves per CU and work order — by timing candidates on the batch itself, and
 persists the winner per (GPU arch, field, shape bucket) in a JSON file
 ($FGF_AUTOTUNE_CACHE, default ~/.cache/fgf_amd/autotune.json). A cached
 entry is reused without timing.
+
+`tune_plan` picks a plan's per-call kernel the same way: the v_perm form at
+several rows per block, and for byte fields the packed 256-entry product
+tables, timed on a dot_product_matrix_with of the plan's shape, cached per
+(GPU arch, field, k, m, row-length bucket).
 """
 from __future__ import annotations
 
 import json
 import math
 import os
+import time
 from statistics import median
 
 CACHE_ENV = "FGF_AUTOTUNE_CACHE"
+MARGIN = 0.95     # a plan keeps its default kernel unless another is 5% faster
 
 
 def cache_path() -> str:
@@ -117,3 +124,90 @@ def tune_batch(batch, reps: int = 3, use_cache: bool = True) -> dict:
         _save(db)
     return {"config": list(best), "cached": False, "key": key, "trials": len(trials),
             "best_ms": round(trials[best], 4), "worst_ms": round(max(trials.values()), 4)}
+
+
+# ---------------------------------------------------------------------------
+# Per-call kernel choice of a plan (fgf_plan_set_kernel).
+# ---------------------------------------------------------------------------
+
+def plan_candidates(nb: int, m: int = 1 << 30) -> list[tuple[int, int]]:
+    """(form, rows per group) choices for an m-row plan: the v_perm form at
+    its widest row group (the default, first) and at narrower ones that
+    split the m rows differently, and for byte fields the packed product
+    tables."""
+    widest = {1: 16, 2: 8, 4: 4, 8: 2}[nb]
+    rows = [0] + [r for r in (8, 4, 2, 1) if r < min(widest, m)][:2]
+    out = [(0, r) for r in rows]
+    if nb == 1:
+        out.append((1, 0))
+    return out
+
+
+def tune_plan(plan, row_len: int, reps: int = 7, use_cache: bool = True) -> dict:
+    """Time every kernel choice of `plan` on a dot_product_matrix_with over
+    synthetic buffers of its shape ((k, m) from the plan, `row_len` bytes per
+    row), set the fastest on the plan, and remember it per (GPU arch, field,
+    k, m, row-length bucket). Outputs of every choice must agree byte for
+    byte. After a warm-up of back-to-back launches the candidates are timed
+    in interleaved rounds (three launches each between events), so clock
+    changes during tuning fall on every candidate alike."""
+    import torch
+
+    from . import ops
+    arch = torch.cuda.get_device_properties(0).gcnArchName.split(":")[0]
+    k, m = plan.dimensions()
+    field = plan.field
+    key = f"{arch}|plan|{field.NAME}|k{k}-m{m}-L{int(math.log2(max(row_len, 1)))}"
+    db = _load() if use_cache else {}
+    if key in db:
+        form, rows = db[key]["choice"]
+        plan.set_kernel(form, rows)
+        return {"choice": [form, rows], "cached": True, "key": key}
+    g = torch.Generator(device="cuda")
+    g.manual_seed(k * 1000 + m)
+    srcs = [torch.randint(0, 256, (row_len,), dtype=torch.uint8, device="cuda", generator=g)
+            for _ in range(k)]
+    out = torch.empty(m * row_len, dtype=torch.uint8, device="cuda")
+    s = torch.cuda.current_stream()
+    cands = plan_candidates(field.BYTES, m)
+    launch = lambda: ops.dot_product_matrix_with(field, out, row_len, m, plan, srcs)
+    ref = None
+    for cand in cands:
+        plan.set_kernel(*cand)
+        launch()
+        torch.cuda.synchronize()
+        if ref is None:
+            ref = out.clone()
+        elif not torch.equal(out, ref):
+            raise RuntimeError(f"kernel choice {cand} disagrees with {cands[0]}")
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < 0.05:       # warm-up: sustained load
+        for _ in range(8):
+            launch()
+        torch.cuda.synchronize()
+    times = {c: [] for c in cands}
+    for _ in range(reps):
+        ev = []
+        for cand in cands:
+            plan.set_kernel(*cand)
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(s)
+            for _ in range(3):
+                launch()
+            b.record(s)
+            ev.append((cand, a, b))
+        torch.cuda.synchronize()
+        for cand, a, b in ev:
+            times[cand].append(a.elapsed_time(b) / 3)
+    trials = {c: median(t) for c, t in times.items()}
+    best = min(trials, key=trials.get)
+    # Hysteresis: leave the default unless a choice beats it by 5%, so a
+    # tie between equally fast kernels does not flip with timing noise.
+    if trials[best] > MARGIN * trials[cands[0]]:
+        best = cands[0]
+    plan.set_kernel(*best)
+    if use_cache:
+        db[key] = {"choice": list(best), "ms": round(trials[best], 4)}
+        _save(db)
+    return {"choice": list(best), "cached": False, "key": key,
+            "trials_ms": {f"form{f}-rows{r}": round(t, 4) for (f, r), t in trials.items()}}

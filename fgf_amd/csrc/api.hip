@@ -163,6 +163,7 @@ struct Tables {
     const uint32_t *dev = nullptr;
     const uint32_t *host = nullptr;
     size_t stride_t = 0, stride_j = 0;
+    const fgf_plan *plan = nullptr;   // a matrix plan's kernel choice applies
 };
 
 Tables plan_tables(const fgf_plan *p, size_t stride_t, size_t stride_j, size_t first = 0) {
@@ -170,6 +171,14 @@ Tables plan_tables(const fgf_plan *p, size_t stride_t, size_t stride_j, size_t f
     t.dev = p->d_tab + first * entry_dw(p->nb);
     t.stride_t = stride_t;
     t.stride_j = stride_j;
+    return t;
+}
+
+// A (k, m) plan's tables in its own matrix layout, coefficient (t, j) at
+// t*m + j: the plan's kernel choice (fgf_plan_set_kernel) applies.
+Tables plan_matrix_tables(const fgf_plan *p) {
+    Tables t = plan_tables(p, p->cols * entry_dw(p->nb), entry_dw(p->nb));
+    t.plan = p;
     return t;
 }
 
@@ -225,6 +234,16 @@ int run_matrix(int f, const Tables &tb, const Rows &rows, const uint8_t *const *
     job.row_len = row_len;
     job.overwrite = overwrite;
     job.stream = s;
+    if (const fgf_plan *p = tb.plan) {
+        job.rows_per_group = p->rows_per_group;
+        // The packed tables hold the plan's own (k, m) layout: only a launch
+        // of that shape can use them.
+        if (p->form == 1 && p->d_ptab && k == p->rows && rows.m == p->cols) {
+            job.ptab = p->d_ptab;
+            job.ptab_rows = p->ptab_rows;
+            job.ptab_k = p->rows;
+        }
+    }
     hipError_t e = launch_matrix(job);
     return e == hipSuccess ? FGF_OK : device_fail("matrix launch", e);
 }
@@ -581,7 +600,48 @@ int fgf_coeff_new(fgf_field f, uint64_t coeff, fgf_plan **out) {
 void fgf_plan_free(fgf_plan *p) {
     if (!p) return;
     (void)hipFree(p->d_tab);
+    if (p->d_ptab) (void)hipFree(p->d_ptab);
     delete p;
+}
+
+int fgf_plan_set_kernel(fgf_plan *p, int form, int rows_per_group) {
+    if (!p) return fail("Plan::set_kernel: plan is null");
+    if (form != 0 && form != 1) return fail("Plan::set_kernel: form %d is not 0 (v_perm) or 1 (packed)", form);
+    if (form == 1 && p->nb != 1)
+        return fail("Plan::set_kernel: packed product tables are built for byte fields, not %s",
+                    field_name(p->field));
+    if (rows_per_group < 0 || rows_per_group > max_rows_for(p->nb))
+        return fail("Plan::set_kernel: %d rows per group outside 0..%d for %s", rows_per_group,
+                    max_rows_for(p->nb), field_name(p->field));
+    if (form == 1 && !p->d_ptab && p->rows && p->cols) {
+        const int R = p->cols > 8 ? 16 : p->cols > 4 ? 8 : 4;
+        std::vector<uint8_t> host(packed_table_bytes(p->rows, p->cols, R));
+        build_packed_tables(p->field, p->values.data(), p->rows, p->cols, R, host.data());
+        uint8_t *d = nullptr;
+        hipError_t e = hipMalloc((void **)&d, host.size());
+        if (e != hipSuccess) {
+            set_errf("hipMalloc: %s", hipGetErrorString(e));
+            return FGF_ENOMEM;
+        }
+        if ((e = hipMemcpy(d, host.data(), host.size(), hipMemcpyHostToDevice)) != hipSuccess) {
+            (void)hipFree(d);
+            return device_fail("hipMemcpy", e);
+        }
+        p->d_ptab = d;
+        p->ptab_rows = R;
+    }
+    // Once built, the packed tables stay with the plan until it is freed,
+    // so switching forms back and forth (as the autotuner does) is free.
+    p->form = form;
+    p->rows_per_group = rows_per_group;
+    return FGF_OK;
+}
+
+int fgf_plan_get_kernel(const fgf_plan *p, int *form, int *rows_per_group) {
+    if (!p) return fail("Plan::get_kernel: plan is null");
+    if (form) *form = p->form;
+    if (rows_per_group) *rows_per_group = p->rows_per_group;
+    return FGF_OK;
 }
 
 size_t fgf_plan_len(const fgf_plan *p) { return p ? p->values.size() : 0; }
@@ -727,8 +787,8 @@ int fgf_mul_add_scatter_with(fgf_field f, uint8_t *rows, size_t rows_len, size_t
                     rows_len, n, row_len, used);
     if (!apart("mul_add_scatter_with", rows, used, &src, &src_len, 1, &rc)) return rc;
     if (all_zero(plan->values) || row_len == 0) return FGF_OK;
-    return run_matrix(f, plan_tables(plan, 0, entry_dw(plan->nb)), rows_contiguous(rows, n, row_len),
-                      &src, 1, row_len, false, S(stream));
+    return run_matrix(f, plan_matrix_tables(plan), rows_contiguous(rows, n, row_len), &src, 1, row_len, false,
+                      S(stream));
 }
 
 namespace {
@@ -856,9 +916,8 @@ int fgf_mul_add_matrix_with(fgf_field f, uint8_t *rows, size_t rows_len, size_t 
         return rc;
     if (!apart("mul_add_matrix_with", rows, used, srcs, src_lens, nsrcs, &rc)) return rc;
     if (nrows == 0 || nsrcs == 0 || row_len == 0) return FGF_OK;
-    const size_t e = entry_dw(plan->nb);
-    return run_matrix(f, plan_tables(plan, nrows * e, e), rows_contiguous(rows, nrows, row_len), srcs,
-                      nsrcs, row_len, false, S(stream));
+    return run_matrix(f, plan_matrix_tables(plan), rows_contiguous(rows, nrows, row_len), srcs, nsrcs,
+                      row_len, false, S(stream));
 }
 
 int fgf_dot_product_matrix(fgf_field f, uint8_t *rows, size_t rows_len, size_t row_len,
@@ -897,9 +956,8 @@ int fgf_dot_product_matrix_with(fgf_field f, uint8_t *rows, size_t rows_len, siz
     if (nrows == 0) return FGF_OK;
     if (nsrcs == 0) return fill_zero(rows, used, S(stream));
     if (row_len == 0) return FGF_OK;
-    const size_t e = entry_dw(plan->nb);
-    return run_matrix(f, plan_tables(plan, nrows * e, e), rows_contiguous(rows, nrows, row_len), srcs,
-                      nsrcs, row_len, true, S(stream));
+    return run_matrix(f, plan_matrix_tables(plan), rows_contiguous(rows, nrows, row_len), srcs, nsrcs,
+                      row_len, true, S(stream));
 }
 
 namespace {
@@ -998,7 +1056,6 @@ int host_pipeline(int f, const char *name, const std::vector<uint8_t *> &hrows, 
     hipStream_t up = p.st[0], enc = p.st[1], down = p.st[2];
     if ((e = hipEventRecord(p.ev_in, s)) != hipSuccess || (e = hipStreamWaitEvent(up, p.ev_in, 0)) != hipSuccess)
         return device_fail("host pipeline ordering", e);
-    const size_t edw = entry_dw(plan->nb);
     std::vector<const uint8_t *> dsrc(k);
     std::vector<uint8_t *> drow(m);
     size_t i = 0;
@@ -1022,8 +1079,8 @@ int host_pipeline(int f, const char *name, const std::vector<uint8_t *> &hrows, 
         if ((e = hipEventRecord(p.ev[0][b], up)) != hipSuccess ||
             (e = hipStreamWaitEvent(enc, p.ev[0][b], 0)) != hipSuccess)
             return device_fail("host pipeline ordering", e);
-        if ((rc = run_matrix(f, plan_tables(plan, m * edw, edw), rows_ptrs(drow.data(), m), dsrc.data(),
-                             k, c, overwrite, enc)))
+        if ((rc = run_matrix(f, plan_matrix_tables(plan), rows_ptrs(drow.data(), m), dsrc.data(), k, c,
+                             overwrite, enc)))
             return rc;
         if ((e = hipEventRecord(p.ev[1][b], enc)) != hipSuccess ||
             (e = hipStreamWaitEvent(down, p.ev[1][b], 0)) != hipSuccess)
@@ -1222,9 +1279,8 @@ int scattered_with(int f, const char *name, uint8_t *dst, size_t dst_len, size_t
             if ((rc = fill_zero(dst + row_starts[j], row_len, s))) return rc;
         return FGF_OK;
     }
-    const size_t e = entry_dw(plan->nb);
-    return run_matrix(f, plan_tables(plan, nrows * e, e), rows_at(dst, row_starts, nrows), srcs, nsrcs,
-                      row_len, overwrite, s);
+    return run_matrix(f, plan_matrix_tables(plan), rows_at(dst, row_starts, nrows), srcs, nsrcs, row_len,
+                      overwrite, s);
 }
 
 }  // namespace

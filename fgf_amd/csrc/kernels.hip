@@ -25,6 +25,22 @@ namespace fgf {
 // One v_perm table set per byte constant of the coefficient (coeff_table).
 void build_coeff_table(int field, uint64_t c, uint32_t *out) { coeff_table(field, c, out); }
 
+void build_packed_tables(int field, const uint64_t *vals, size_t k, size_t m, int R, uint8_t *out) {
+    const size_t groups = (m + R - 1) / R;
+    for (size_t g = 0; g < groups; ++g)
+        for (size_t t = 0; t < k; ++t) {
+            uint8_t *rec = out + (g * k + t) * 256u * R;
+            for (int j = 0; j < R; ++j) {
+                const size_t row = g * R + j;
+                uint32_t e[TAB_DW];
+                byte_const_table(field, row < m ? (uint8_t)vals[t * m + row] : 0, e);
+                // l * x from the v_perm tables: T0[x & 7] ^ T1[(x >> 3) & 7] ^ T2[x >> 6].
+                const uint8_t *t0 = (const uint8_t *)&e[0], *t1 = (const uint8_t *)&e[2], *t2 = (const uint8_t *)&e[4];
+                for (unsigned x = 0; x < 256; ++x) rec[x * R + j] = t0[x & 7] ^ t1[(x >> 3) & 7] ^ t2[x >> 6];
+            }
+        }
+}
+
 namespace {
 
 using namespace core;
@@ -163,7 +179,7 @@ __global__ __launch_bounds__(BLOCK) void k_matrix_generic(const MatrixArgs a) {
 using MatrixFn = void (*)(MatrixArgs);
 
 // Rows per block for each element width, bounded by the accumulator budget.
-constexpr int max_rows(int nb) { return nb == 1 ? 16 : nb == 2 ? 8 : nb == 4 ? 4 : 2; }
+constexpr int max_rows(int nb) { return max_rows_for(nb); }
 // Dynamic LDS one block may use for tables on the current device: the
 // smaller of LDS_TABLE_BYTES and the device's per-block limit.
 size_t lds_budget() {
@@ -265,7 +281,8 @@ hipError_t launch_segment(const MatrixJob &job, MatrixArgs a, size_t k0, size_t 
     if (vector) {
         // Every block handles exactly mt rows; a ragged last row group gets a
         // launch of its own with its own compile-time row count.
-        const int mt = std::min<int>((int)mn, max_rows(nb));
+        const int cap = job.rows_per_group > 0 ? std::min(job.rows_per_group, max_rows(nb)) : max_rows(nb);
+        const int mt = std::min<int>((int)mn, cap);
         const size_t full = (mn / mt) * mt;
         if (full < mn) {
             hipError_t e = launch_segment(job, a, k0, kn, j0 + full, mn - full, col0, ncols, vector, ow);
@@ -294,6 +311,164 @@ hipError_t launch_segment(const MatrixJob &job, MatrixArgs a, size_t k0, size_t 
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------------------
+// Packed product-table form (byte fields). Per (row group of R = 4, 8 or 16
+// rows, source)
+// a 256-entry table whose record x is the R products c(t, j) * x, so one
+// random-index ds_read of R bytes applies a source byte to every row of the
+// group. Accumulators are position-major: acc[p] holds the R row bytes of
+// column byte p; a 4 x 4 byte transpose per dword turns them into rows at
+// the end. Tables are staged into LDS PTAB_STAGE sources at a time.
+// ---------------------------------------------------------------------------
+
+constexpr size_t PTAB_LDS = 32 * 1024;
+// Threads per block: the waves of a block share its staged tables.
+constexpr int PTAB_BLOCK = 512;
+
+struct PtabArgs {
+    const uint8_t *src[KMAX];
+    uint8_t *dst[MMAX];
+    const uint8_t *ptab;        // records of source 0 of this launch, row group 0 of this launch
+    uint64_t group_stride;      // bytes between row groups in the plan's packed tables
+    uint64_t col0, ncols;
+    uint32_t k, m;              // sources and rows of this launch
+    uint32_t ngroups, ncoltiles;
+};
+
+template <int R, bool OW>
+__global__ __launch_bounds__(PTAB_BLOCK) void k_matrix_ptab(const PtabArgs a) {
+    constexpr int Q = R / 4;                                  // dwords per record
+    constexpr uint32_t STAGE = (uint32_t)(PTAB_LDS / (256u * R));  // sources per LDS stage
+    extern __shared__ uint4 s_pt[];
+    const uint32_t b = blockIdx.x;
+    const uint32_t rg = __builtin_amdgcn_readfirstlane(b % a.ngroups);
+    const uint32_t ct = __builtin_amdgcn_readfirstlane(b / a.ngroups);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint64_t tile = ((uint64_t)ct * (PTAB_BLOCK / 64) + wave) * 1024u;
+    // Waves past the columns keep running with empty resources: every wave
+    // takes part in the staging barriers.
+    const uint32_t nb = tile >= a.ncols ? 0u : (uint32_t)std::min<uint64_t>(1024u, a.ncols - tile);
+    const uint64_t base = a.col0 + tile;
+    const uint32_t lo = lane * 16u;
+    const uint32_t j0 = rg * R;
+    const uint32_t mrows = std::min<uint32_t>(R, a.m - j0);
+    const uint8_t *pt = a.ptab + rg * a.group_stride;
+
+    uint32_t acc[16][Q];
+    if constexpr (OW) {
+#pragma unroll
+        for (int p = 0; p < 16; ++p)
+#pragma unroll
+            for (int q = 0; q < Q; ++q) acc[p][q] = 0;
+    } else {
+        // Rows 4q..4q+3, dword d (columns 4d..4d+3) -> acc[4d..4d+3][q].
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            uint32_t w[4][4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const uint32_t j = 4 * q + i;
+                const BufRes r = buf_res(j < mrows ? (const uint8_t *)a.dst[j0 + j] + base : nullptr,
+                                         j < mrows ? nb : 0u);
+                bload16(w[i], r, lo);
+            }
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                uint32_t x0 = w[0][d], x1 = w[1][d], x2 = w[2][d], x3 = w[3][d];
+                transpose4(x0, x1, x2, x3);
+                acc[4 * d][q] = x0;
+                acc[4 * d + 1][q] = x1;
+                acc[4 * d + 2][q] = x2;
+                acc[4 * d + 3][q] = x3;
+            }
+        }
+    }
+
+    auto load_src = [&](uint32_t (&x)[4], uint32_t t) {
+        const bool real = t < a.k;
+        bload16(x, buf_res(real ? (const uint8_t *)a.src[t] + base : nullptr, real ? nb : 0u), lo);
+    };
+    uint32_t xc[4], xn[4];
+    load_src(xc, 0);
+    for (uint32_t t = 0; t < a.k; ++t) {
+        const uint32_t tl = t % STAGE;
+        if (tl == 0) {
+            const uint32_t kn = std::min<uint32_t>(STAGE, a.k - t);
+            const uint4 *g = (const uint4 *)(pt + (size_t)t * 256u * R);
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < kn * 16u * R; i += PTAB_BLOCK) s_pt[i] = g[i];
+            __syncthreads();
+        }
+        load_src(xn, t + 1);
+#pragma unroll
+        for (int p = 0; p < 16; ++p) {
+            const uint32_t x = __builtin_amdgcn_ubfe(xc[p >> 2], 8u * (p & 3), 8u);
+            if constexpr (R == 16) {
+                const uint4 e = s_pt[tl * 256u + x];
+                acc[p][0] ^= e.x;
+                acc[p][1] ^= e.y;
+                acc[p][2] ^= e.z;
+                acc[p][3] ^= e.w;
+            } else if constexpr (R == 8) {
+                const uint2 e = ((const uint2 *)s_pt)[tl * 256u + x];
+                acc[p][0] ^= e.x;
+                acc[p][1] ^= e.y;
+            } else {
+                acc[p][0] ^= ((const uint32_t *)s_pt)[tl * 256u + x];
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < 4; ++d) xc[d] = xn[d];
+    }
+
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        uint32_t w[4][4];
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            uint32_t x0 = acc[4 * d][q], x1 = acc[4 * d + 1][q], x2 = acc[4 * d + 2][q], x3 = acc[4 * d + 3][q];
+            transpose4(x0, x1, x2, x3);
+            w[0][d] = x0;
+            w[1][d] = x1;
+            w[2][d] = x2;
+            w[3][d] = x3;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t j = 4 * q + i;
+            if (j < mrows) bstore16(buf_res(a.dst[j0 + j] + base, nb), lo, w[i]);
+        }
+    }
+}
+
+// The packed-table launches of one column segment (all pointers + col0
+// 16-byte aligned, ncols a multiple of 16): sources k0..k0+kn of the plan,
+// rows j0..j0+mn (j0 a multiple of 16), groups of R rows.
+hipError_t launch_ptab_segment(const MatrixJob &job, size_t k0, size_t kn, size_t j0, size_t mn,
+                               uint64_t col0, uint64_t ncols, bool ow) {
+    if (ncols == 0) return hipSuccess;
+    const int R = job.ptab_rows;
+    PtabArgs a{};
+    for (size_t t = 0; t < kn; ++t) a.src[t] = job.srcs[k0 + t];
+    for (size_t j = 0; j < mn; ++j) a.dst[j] = job.row(j0 + j);
+    a.group_stride = (uint64_t)job.ptab_k * 256u * R;
+    a.ptab = job.ptab + (j0 / R) * a.group_stride + k0 * 256u * R;
+    a.col0 = col0;
+    a.ncols = ncols;
+    a.k = (uint32_t)kn;
+    a.m = (uint32_t)mn;
+    a.ngroups = (uint32_t)((mn + R - 1) / R);
+    a.ncoltiles = (uint32_t)((ncols + PTAB_BLOCK * 16 - 1) / (PTAB_BLOCK * 16));
+    const uint64_t blocks = (uint64_t)a.ncoltiles * a.ngroups;
+    using Fn = void (*)(PtabArgs);
+    const Fn fn = R == 16  ? (ow ? (Fn)k_matrix_ptab<16, true> : (Fn)k_matrix_ptab<16, false>)
+                  : R == 8 ? (ow ? (Fn)k_matrix_ptab<8, true> : (Fn)k_matrix_ptab<8, false>)
+                           : (ow ? (Fn)k_matrix_ptab<4, true> : (Fn)k_matrix_ptab<4, false>);
+    hipLaunchKernelGGL(fn, dim3((uint32_t)blocks), dim3(PTAB_BLOCK), PTAB_LDS, job.stream, a);
+    return hipGetLastError();
+}
 
 // Columns per launch: a multiple of every block's column span and element
 // size; 8 GiB keeps the generic kernel at 2^31 work-items.
@@ -333,7 +508,14 @@ hipError_t launch_matrix(const MatrixJob &job) {
             };
             hipError_t e;
             if ((e = segments(0, head, false)) != hipSuccess) return e;
-            if ((e = segments(head, body, true)) != hipSuccess) return e;
+            if (job.ptab && nb == 1) {
+                for (uint64_t c = 0; c < body; c += MAX_SEGMENT)
+                    if ((e = launch_ptab_segment(job, k0, kn, j0, mn, head + c,
+                                                 std::min<uint64_t>(MAX_SEGMENT, body - c), ow)) != hipSuccess)
+                        return e;
+            } else if ((e = segments(head, body, true)) != hipSuccess) {
+                return e;
+            }
             if ((e = segments(tail0, tail, false)) != hipSuccess) return e;
         }
     }

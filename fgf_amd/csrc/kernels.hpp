@@ -61,6 +61,13 @@ struct MatrixJob {
     size_t row_len = 0;
     bool overwrite = false;
     hipStream_t stream = nullptr;
+    // Kernel choice (a plan's tuning): rows per block of the v_perm form
+    // (0: the most the field allows), or the packed product tables of a
+    // byte-field plan (ptab non-null: see build_packed_tables).
+    int rows_per_group = 0;
+    const uint8_t *ptab = nullptr;
+    int ptab_rows = 0;                     // R: 4, 8 or 16
+    size_t ptab_k = 0;                     // sources of the plan (group stride)
 
     uint8_t *row(size_t j) const {
         return rows ? rows[j] : row_base + (row_offsets ? row_offsets[j] : j * row_stride);
@@ -113,6 +120,17 @@ FGF_HD constexpr void coeff_table(int f, uint64_t c, uint32_t *out) {
 
 // Host-side table builder: writes the entry_dwords(B) dwords of coefficient c.
 void build_coeff_table(int field, uint64_t c, uint32_t *out);
+
+// Packed product tables of a byte-field (k, m) plan: rows in groups of R
+// (4, 8 or 16), and for group g and source t a 256-entry table whose record x
+// (R bytes) holds c(t, g*R + j) * x for j < R (zero past row m). Record x of
+// (g, t) is at byte ((g * k + t) * 256 + x) * R. `vals` are the plan values
+// (coefficient (t, j) at t*m + j); `out` holds packed_table_bytes(k, m, R).
+inline size_t packed_table_bytes(size_t k, size_t m, int R) { return ((m + R - 1) / R) * k * 256 * (size_t)R; }
+void build_packed_tables(int field, const uint64_t *vals, size_t k, size_t m, int R, uint8_t *out);
+
+// Rows per block of the v_perm form the field allows (the default).
+constexpr int max_rows_for(int nb) { return nb == 1 ? 16 : nb == 2 ? 8 : nb == 4 ? 4 : 2; }
 
 // Sets the thread-local text returned by fgf_last_error().
 void set_last_error(const char *msg);

@@ -139,6 +139,18 @@ class Plan:
             lib.fgf_plan_free(h)
             self._h = None
 
+    def set_kernel(self, form: int = 0, rows_per_group: int = 0) -> None:
+        """Kernel choice for this plan's per-call matrix launches (no
+        counterpart in the reference): form 0 = v_perm byte-constant tables,
+        1 = packed 256-entry product tables (byte fields); rows_per_group 0 =
+        the field's maximum. See fgf_plan_set_kernel in include/fgf_amd.h."""
+        _check(_lib.lib().fgf_plan_set_kernel(self._h, int(form), int(rows_per_group)))
+
+    def kernel(self) -> tuple[int, int]:
+        form, rows = ctypes.c_int(), ctypes.c_int()
+        _check(_lib.lib().fgf_plan_get_kernel(self._h, ctypes.byref(form), ctypes.byref(rows)))
+        return form.value, rows.value
+
     def __len__(self) -> int:
         return _lib.lib().fgf_plan_len(self._h)
 

@@ -117,6 +117,21 @@ void fgf_plan_dimensions(const fgf_plan *plan, size_t *rows, size_t *cols);
 fgf_field fgf_plan_field(const fgf_plan *plan);
 int fgf_plan_get(const fgf_plan *plan, size_t index, uint64_t *value);
 int fgf_plan_get_at(const fgf_plan *plan, size_t row, size_t col, uint64_t *value);
+/* Kernel choice for the per-call matrix launches of a (k, m) plan (the
+ * `_with` matrix, scatter and scattered forms, device and host buffers); no
+ * counterpart in the reference, whose backend picks one form per CPU. The
+ * autotuner (fgf_amd/autotune.py tune_plan) times the choices and sets the
+ * fastest. form 0: v_perm byte-constant tables (the default). form 1 (byte
+ * fields only): packed 256-entry product tables, one random-index LDS read
+ * per source byte carrying the products of 4, 8 or 16 rows; selecting it builds
+ * and uploads them (k * ceil(m / R) * 256 * R bytes, R = 16 if m > 8, 8 if
+ * m > 4, else 4), kept with the plan until it is freed.
+ * rows_per_group: destination rows per block of the v_perm form, 1 up to
+ * 16 / 8 / 4 / 2 for 1- / 2- / 4- / 8-byte fields, 0 for that maximum (the
+ * default). Set before the plan is shared: not concurrently with calls that
+ * use it. */
+int fgf_plan_set_kernel(fgf_plan *plan, int form, int rows_per_group);
+int fgf_plan_get_kernel(const fgf_plan *plan, int *form, int *rows_per_group);
 
 /* ------------------------------------------------------------------------
  * Single-row operations (src/ops.rs:323-450).
