@@ -289,12 +289,10 @@ def bench_batched(n_stripes: int = 4096, seed: int = 4096, reps: int = 5) -> dic
     total = sum(sp.algorithmic_bytes for sp in specs)
     stream = torch.cuda.current_stream()
 
-    def timed(fn, warm=3):
-        # A few untimed launches first: after the host-bound per-call leg the
-        # GPU's clocks take a few milliseconds of load to come back up.
-        for _ in range(warm):
-            fn()
-        torch.cuda.synchronize()
+    def timed(fn):
+        # A pre-roll first: after the host-bound per-call leg (or idle time)
+        # the GPU's clocks take some tens of ms of load to come back up.
+        _preroll(fn, 100.0)
         times = []
         for _ in range(reps):
             a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -381,14 +379,26 @@ def profile_batched(launches: int, n_stripes: int = 4096, seed: int = 4096,
             "GBps": round(total / (per[len(per) // 2] * 1e-3) / 1e9, 1)}
 
 
-def _median_launch_ms(fn, reps: int = 20, warm: int = 3) -> float:
+def _preroll(fn, ms: float) -> None:
+    """Back-to-back launches for at least `ms` of wall clock, so what is
+    timed next runs at sustained clocks (an idle GPU takes some tens of ms of
+    load to settle)."""
+    import torch
+    t0 = time.perf_counter()
+    n = 1
+    while (time.perf_counter() - t0) * 1e3 < ms:
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        n = min(2 * n, 64)
+
+
+def _median_launch_ms(fn, reps: int = 20, preroll_ms: float = 100.0) -> float:
     """Median per-launch time from HIP events recorded between back-to-back
-    launches on the current stream."""
+    launches on the current stream, after a pre-roll."""
     import torch
     stream = torch.cuda.current_stream()
-    for _ in range(warm):
-        fn()
-    torch.cuda.synchronize()
+    _preroll(fn, preroll_ms)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
     ev[0].record(stream)
     for i in range(reps):
@@ -405,9 +415,7 @@ def _back_to_back_ms(fn, n: int = 50, reps: int = 5) -> float:
     calls in a loop, without an event packet between launches."""
     import torch
     stream = torch.cuda.current_stream()
-    for _ in range(3):
-        fn()
-    torch.cuda.synchronize()
+    _preroll(fn, 50.0)
     per = []
     for _ in range(reps):
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

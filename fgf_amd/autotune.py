@@ -130,14 +130,21 @@ def tune_batch(batch, reps: int = 3, use_cache: bool = True) -> dict:
 # Per-call kernel choice of a plan (fgf_plan_set_kernel).
 # ---------------------------------------------------------------------------
 
+DEFAULT_ROWS = {1: 16, 2: 8, 4: 4, 8: 4}   # rows per block with rows_per_group 0
+MAX_ROWS = {1: 16, 2: 8, 4: 4, 8: 8}
+
+
 def plan_candidates(nb: int, m: int = 1 << 30) -> list[tuple[int, int]]:
     """(form, rows per group) choices for an m-row plan: the v_perm form at
-    its widest row group (the default, first) and at narrower ones that
-    split the m rows differently, and for byte fields the packed product
-    tables."""
-    widest = {1: 16, 2: 8, 4: 4, 8: 2}[nb]
-    rows = [0] + [r for r in (8, 4, 2, 1) if r < min(widest, m)][:2]
-    out = [(0, r) for r in rows]
+    its default row group (first), at the widest and at half the default,
+    dropping choices that split the m rows the same way, and for byte
+    fields the packed product tables."""
+    default = DEFAULT_ROWS[nb]
+    out, seen = [(0, 0)], {min(default, m)}
+    for r in (MAX_ROWS[nb], default // 2):
+        if r >= 1 and min(r, m) not in seen:
+            seen.add(min(r, m))
+            out.append((0, r))
     if nb == 1:
         out.append((1, 0))
     return out

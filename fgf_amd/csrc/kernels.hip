@@ -73,6 +73,11 @@ constexpr size_t LDS_TABLE_BYTES = 76 * 1024;
 // T2 dwords go to LDS too when a block's kterms(B)*M scalars per source would
 // crowd the SGPRs (towers, and byte fields with more than 8 rows).
 constexpr bool t2_lds(int nb, int m) { return nb > 1 || m > 8; }
+// 8-byte towers stage their tables inside the source loop, 64 / M sources
+// (rounded down to a multiple of 4) at a time for M rows
+// (matrix_tile_staged): at most 64 x 27 byte constants x 20 B = 34,560 B of
+// LDS per block at every row count, one stage for k <= 32 at two rows.
+constexpr uint32_t stage8(int m) { return (64u / (uint32_t)m) & ~3u; }
 constexpr size_t lds_record_bytes(int nb, int m) { return (size_t)kterms(nb) * (t2_lds(nb, m) ? 20 : 16); }
 
 // Block index -> (column tile, row group). Blocks are dealt to the XCDs round
@@ -100,14 +105,15 @@ __device__ __forceinline__ bool map_block(const MatrixArgs &a, uint32_t &ct, uin
 // multiple of 16.
 //
 // The block first stages its rows' tables in LDS (dynamic shared memory, at
-// most LDS_TABLE_BYTES), then every wave runs its tile.
-// GF(2^64) asks for three waves per SIMD: its 81 Karatsuba selector
-// registers would otherwise leave two, too few to cover its 27 table reads
-// per source and row (measured 11% faster at two rows per block). GF(2^16)
-// at 7-8 rows and accumulating GF(2^8) at 13-16 rows sit a few registers
-// above the three-wave limit of 168 VGPRs and are held to it as well.
+// most LDS_TABLE_BYTES), then every wave runs its tile; 8-byte towers stage
+// them in chunks inside the source loop instead (matrix_tile_staged).
+// GF(2^64) with up to four rows asks for three waves per SIMD (146-157
+// VGPRs with the branch-at-a-time expansion of consume_split); eight rows
+// take 218 and run at two. GF(2^16) at 7-8 rows and accumulating GF(2^8) at
+// 13-16 rows sit a few registers above the three-wave limit of 168 VGPRs
+// and are held to it.
 constexpr int matrix_waves_per_eu(int nb, int m, bool ow) {
-    return nb == 8 || (nb == 2 && m >= 7) || (nb == 1 && m >= 13 && !ow) ? 3 : 1;
+    return (nb == 8 && m <= 4) || (nb == 2 && m >= 7) || (nb == 1 && m >= 13 && !ow) ? 3 : 1;
 }
 
 template <int B, bool FP, int M, int D, bool OW>
@@ -121,10 +127,16 @@ void k_matrix(const MatrixArgs a) {
     const uint32_t j0 = rg * M;
     const uint32_t *tab = table_of(a) + (size_t)j0 * a.tab_stride_j;
     extern __shared__ uint4 s_lt[];
-    const auto tb = stage_lds_table<B, t2_lds(B, M)>(s_lt, tab, a.k, M, a.tab_stride_t,
-                                                  a.tab_stride_j);
-    if (tile >= a.ncols) return;
-    matrix_tile<B, M, D, FP>(a.src, a.dst + j0, tb, a.k, OW, a.col0, a.ncols, tile, lane);
+    if constexpr (B == 8) {
+        // Every wave runs the staged loop (its barriers), past ncols or not.
+        matrix_tile_staged<B, M, D, FP, stage8(M)>(a.src, a.dst + j0, s_lt, tab, a.tab_stride_t,
+                                                   a.tab_stride_j, a.k, OW, a.col0, a.ncols, tile, lane);
+    } else {
+        const auto tb = stage_lds_table<B, t2_lds(B, M)>(s_lt, tab, a.k, M, a.tab_stride_t,
+                                                      a.tab_stride_j);
+        if (tile >= a.ncols) return;
+        matrix_tile<B, M, D, FP>(a.src, a.dst + j0, tb, a.k, OW, a.col0, a.ncols, tile, lane);
+    }
 }
 
 // Generic kernel for heads, tails and unaligned buffers: one thread per group
@@ -201,6 +213,7 @@ size_t lds_budget() {
 // Sources per launch: KMAX, or fewer when a full row group's LDS tables would
 // exceed the LDS budget.
 size_t k_chunk(int nb) {
+    if (nb == 8) return KMAX;  // staged in chunks of stage8(rows) inside the kernel
     const size_t per_source = (size_t)max_rows(nb) * lds_record_bytes(nb, max_rows(nb));
     return std::max<size_t>(1, std::min<size_t>(KMAX, lds_budget() / per_source));
 }
@@ -281,7 +294,7 @@ hipError_t launch_segment(const MatrixJob &job, MatrixArgs a, size_t k0, size_t 
     if (vector) {
         // Every block handles exactly mt rows; a ragged last row group gets a
         // launch of its own with its own compile-time row count.
-        const int cap = job.rows_per_group > 0 ? std::min(job.rows_per_group, max_rows(nb)) : max_rows(nb);
+        const int cap = job.rows_per_group > 0 ? std::min(job.rows_per_group, max_rows(nb)) : default_rows_for(nb);
         const int mt = std::min<int>((int)mn, cap);
         const size_t full = (mn / mt) * mt;
         if (full < mn) {
@@ -297,7 +310,7 @@ hipError_t launch_segment(const MatrixJob &job, MatrixArgs a, size_t k0, size_t 
         const uint64_t blocks = a.ngroups == 1
                                     ? a.ncoltiles
                                     : (uint64_t)((a.ncoltiles + 7) / 8) * 8u * a.ngroups;
-        const size_t lds = kn * (size_t)mt * lds_record_bytes(nb, mt);
+        const size_t lds = (nb == 8 ? std::min<size_t>(kn, stage8(mt)) : kn) * (size_t)mt * lds_record_bytes(nb, mt);
         hipLaunchKernelGGL(pick_vector(nb, job.fan_paar, mt, ow), dim3((uint32_t)blocks), dim3(BLOCK), lds,
                            job.stream, a);
     } else {

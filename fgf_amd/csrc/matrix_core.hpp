@@ -547,6 +547,122 @@ __device__ __forceinline__ void matrix_tile(SrcPtrs src, DstPtrs dst, const Tab 
     }
 }
 
+// Towers of 8-byte elements with many rows per wave. The 27 byte constants
+// of a coefficient form three Karatsuba branches of 9 (S, O1, O2 at the top
+// level) acting on the half-element planes a (bytes 0-3), b (bytes 4-7) and
+// a ^ b: towers S on a, O1 on b, O2 on a ^ b; Fan-Paar S on a ^ b, O1 on a,
+// O2 on b. One branch at a time, its 9 planes' selectors (27 registers
+// instead of 81) are expanded once and shared by every row, and each row's
+// 9 products combine into a half product that goes to the low half (S, O1)
+// and the high half (S, O2) of the row's accumulator: lo = S ^ O1,
+// hi = S ^ O2, as kara_accumulate<8> does in one step.
+template <int B, int M, int D, bool FP, typename Tab>
+__device__ __forceinline__ void consume_split(uint32_t (&acc)[M][D], const uint32_t *x, const Tab &tb) {
+    static_assert(B == 8 && D == B, "one group of four 8-byte elements per lane");
+    constexpr int H = B / 2, EH = kterms(H);
+    uint32_t pl[B];
+    to_planes<B>(x, pl);
+#pragma unroll
+    for (int br = 0; br < 3; ++br) {
+        const int which = FP ? (br == 0 ? 2 : br - 1) : br;  // 0: a, 1: b, 2: a ^ b
+        Sel half[H];
+#pragma unroll
+        for (int q = 0; q < H; ++q)
+            half[q] = selectors(which == 0 ? pl[q] : which == 1 ? pl[H + q] : (pl[q] ^ pl[H + q]));
+        Sel sel[EH];
+        kara_expand<H, FP>(half, sel);
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            uint32_t lk[EH];
+#pragma unroll
+            for (int i = 0; i < EH; ++i) lk[i] = tb.lookup(0, j, br * EH + i, sel[i]);
+            uint32_t out[H];
+            kara_combine<H>(lk, out);
+#pragma unroll
+            for (int q = 0; q < H; ++q) {
+                if (br != 2) acc[j][q] ^= out[q];
+                if (br != 1) acc[j][H + q] ^= out[q];
+            }
+        }
+    }
+}
+
+// matrix_tile for 8-byte towers with all M rows of the block in one wave
+// (consume_split), the tables staged into LDS KC sources at a time inside
+// the source loop, so one launch takes any number of sources without the
+// block's LDS growing with k. Every wave of the block runs the whole loop
+// (a wave past ncols with empty resources) and takes part in every staging
+// barrier. KC is a multiple of 4, so each pass of four sources lies in one
+// chunk.
+template <int B, int M, int D, bool FP, uint32_t KC, typename SrcPtrs, typename DstPtrs>
+__device__ __forceinline__ void matrix_tile_staged(SrcPtrs src, DstPtrs dst, void *lds,
+                                                   const uint32_t *__restrict__ tab, uint32_t stride_t,
+                                                   uint32_t stride_j, uint32_t k, bool ow, uint64_t col0,
+                                                   uint64_t ncols, uint64_t tile, uint32_t lane) {
+    constexpr int P = D / 4;
+    constexpr uint32_t CHUNK = 256u * D;
+    static_assert(KC % 4 == 0 && D == B, "tile shape");  // M <= 8 rows: KC = 64 / M >= 8
+    uint32_t lo[P];
+#pragma unroll
+    for (int i = 0; i < P; ++i) lo[i] = (uint32_t)i * 1024u + lane * 16u;
+    const uint32_t nb = tile >= ncols ? 0u : ncols - tile < CHUNK ? (uint32_t)(ncols - tile) : CHUNK;
+    const uint64_t base = col0 + tile;
+
+    auto load_src = [&](uint32_t (&x)[D], uint32_t t) {
+        const bool real = t < k;
+        const BufRes r = buf_res((const uint8_t *)src[real ? t : 0u] + base, real ? nb : 0u);
+#pragma unroll
+        for (int i = 0; i < P; ++i) bload16(&x[4 * i], r, lo[i]);
+    };
+
+    uint32_t acc[M][D];
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        uint32_t w[D];
+        if (ow) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) w[d] = 0;
+        } else {
+            const BufRes r = buf_res((const uint8_t *)dst[j] + base, nb);
+#pragma unroll
+            for (int i = 0; i < P; ++i) bload16(&w[4 * i], r, lo[i]);
+        }
+        to_planes<B>(w, acc[j]);
+    }
+
+    uint32_t xa[2][D], xb[2][D];
+    load_src(xa[0], 0);
+    load_src(xa[1], 1);
+    LdsTab<B, true> tb{};
+    uint32_t t = 0;
+    do {
+        const uint32_t tl = t % KC;
+        if (tl == 0) {
+            __syncthreads();  // every wave is done with the previous chunk's tables
+            tb = stage_lds_table<B, true>(lds, tab + (size_t)t * stride_t, k - t < KC ? k - t : KC, M,
+                                          stride_t, stride_j);
+        }
+        load_src(xb[0], t + 2);
+        load_src(xb[1], t + 3);
+        consume_split<B, M, D, FP>(acc, xa[0], tb.at(tl));
+        if (t + 1 < k) consume_split<B, M, D, FP>(acc, xa[1], tb.at(tl + 1));
+        load_src(xa[0], t + 4);
+        load_src(xa[1], t + 5);
+        if (t + 2 < k) consume_split<B, M, D, FP>(acc, xb[0], tb.at(tl + 2));
+        if (t + 3 < k) consume_split<B, M, D, FP>(acc, xb[1], tb.at(tl + 3));
+        t += 4;
+    } while (t < k);
+
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        uint32_t w[D];
+        from_planes<B>(acc[j], w);
+        const BufRes r = buf_res((uint8_t *)dst[j] + base, nb);
+#pragma unroll
+        for (int i = 0; i < P; ++i) bstore16(r, lo[i], &w[4 * i]);
+    }
+}
+
 // A run of nsub consecutive wave tiles of one operation (tile0, tile0 +
 // 256*D, ...), rows[j] (=|^=) sum_t c(t, j) * src[t], as ONE software
 // pipeline. Each tile is a sequence of steps: the k sources, then (when

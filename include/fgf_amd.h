@@ -127,9 +127,9 @@ int fgf_plan_get_at(const fgf_plan *plan, size_t row, size_t col, uint64_t *valu
  * and uploads them (k * ceil(m / R) * 256 * R bytes, R = 16 if m > 8, 8 if
  * m > 4, else 4), kept with the plan until it is freed.
  * rows_per_group: destination rows per block of the v_perm form, 1 up to
- * 16 / 8 / 4 / 2 for 1- / 2- / 4- / 8-byte fields, 0 for that maximum (the
- * default). Set before the plan is shared: not concurrently with calls that
- * use it. */
+ * 16 / 8 / 4 / 8 for 1- / 2- / 4- / 8-byte fields, 0 for the default (16 /
+ * 8 / 4 / 4). Set before the plan is shared: not concurrently with calls
+ * that use it. */
 int fgf_plan_set_kernel(fgf_plan *plan, int form, int rows_per_group);
 int fgf_plan_get_kernel(const fgf_plan *plan, int *form, int *rows_per_group);
 

@@ -88,7 +88,7 @@ def test_packed_tables_scattered_and_misaligned(f):
 
 @pytest.mark.parametrize("f", [Gf8D, Gf16, Gf32, Gf64], ids=lambda f: NAMES[f.id])
 def test_row_groups_match_default(f):
-    k, m, L = 12, {1: 16, 2: 8, 4: 4, 8: 2}[f.BYTES] + 1, 8192 + 64
+    k, m, L = 12, {1: 16, 2: 8, 4: 4, 8: 8}[f.BYTES] + 1, 8192 + 64
     vals = coeffs(f, k * m, 0x700 + f.id)
     plan = ops.Plan.matrix(f, k, m, vals)
     srcs = [ol.noise(L, 0x800 + t) for t in range(k)]
