@@ -27,6 +27,26 @@ void build_coeff_table(int field, uint64_t c, uint32_t *out) { coeff_table(field
 
 void build_packed_tables(int field, const uint64_t *vals, size_t k, size_t m, int R, uint8_t *out) {
     const size_t groups = (m + R - 1) / R;
+    if (field_bytes(field) == 2) {
+        // Two tables per (group, source): byte position i of the element,
+        // record x = the R two-byte products c(t, j) * (x << 8i).
+        const size_t RB = 2 * (size_t)R;
+        for (size_t g = 0; g < groups; ++g)
+            for (size_t t = 0; t < k; ++t)
+                for (int j = 0; j < R; ++j) {
+                    const size_t row = g * R + j;
+                    const uint64_t c = row < m ? vals[t * m + row] : 0;
+                    for (int i = 0; i < 2; ++i) {
+                        uint8_t *tab = out + ((g * k + t) * 2 + i) * 256u * RB;
+                        for (unsigned x = 0; x < 256; ++x) {
+                            const uint64_t p = HostField::mul(field, c, (uint64_t)x << (8 * i));
+                            tab[x * RB + 2 * j] = (uint8_t)p;
+                            tab[x * RB + 2 * j + 1] = (uint8_t)(p >> 8);
+                        }
+                    }
+                }
+        return;
+    }
     for (size_t g = 0; g < groups; ++g)
         for (size_t t = 0; t < k; ++t) {
             uint8_t *rec = out + (g * k + t) * 256u * R;
@@ -456,6 +476,108 @@ __global__ __launch_bounds__(PTAB_BLOCK) void k_matrix_ptab(const PtabArgs a) {
     }
 }
 
+// The packed form for 2-byte elements (GF(2^16), Fan-Paar 16): per (group of
+// R rows, source) two 256-entry tables, one per byte position i of the
+// element, whose record x holds the R two-byte products c(t, j) * (x << 8i).
+// A lane's 16 column bytes are 8 elements; element e takes one lookup per
+// byte and acc[e] holds its R two-byte row products (rows 2q, 2q + 1 in
+// dword q). At the end a v_perm pair per dword moves them into rows.
+template <int R, bool OW>
+__global__ __launch_bounds__(PTAB_BLOCK) void k_matrix_ptab2(const PtabArgs a) {
+    constexpr int RB = 2 * R;                                  // record bytes
+    constexpr int Q = RB / 4;                                  // dwords per record
+    constexpr uint32_t STAGE = (uint32_t)(PTAB_LDS / (512u * RB));  // sources per LDS stage
+    extern __shared__ uint4 s_pt[];
+    const uint32_t b = blockIdx.x;
+    const uint32_t rg = __builtin_amdgcn_readfirstlane(b % a.ngroups);
+    const uint32_t ct = __builtin_amdgcn_readfirstlane(b / a.ngroups);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint64_t tile = ((uint64_t)ct * (PTAB_BLOCK / 64) + wave) * 1024u;
+    const uint32_t nb = tile >= a.ncols ? 0u : (uint32_t)std::min<uint64_t>(1024u, a.ncols - tile);
+    const uint64_t base = a.col0 + tile;
+    const uint32_t lo = lane * 16u;
+    const uint32_t j0 = rg * R;
+    const uint32_t mrows = std::min<uint32_t>(R, a.m - j0);
+    const uint8_t *pt = a.ptab + rg * a.group_stride;
+
+    uint32_t acc[8][Q];
+    if constexpr (OW) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+#pragma unroll
+            for (int q = 0; q < Q; ++q) acc[e][q] = 0;
+    } else {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            uint32_t w0[4], w1[4];   // rows 2q and 2q + 1
+            const uint32_t j = 2 * q;
+            bload16(w0, buf_res(j < mrows ? (const uint8_t *)a.dst[j0 + j] + base : nullptr, j < mrows ? nb : 0u), lo);
+            bload16(w1, buf_res(j + 1 < mrows ? (const uint8_t *)a.dst[j0 + j + 1] + base : nullptr,
+                                j + 1 < mrows ? nb : 0u), lo);
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                acc[2 * d][q] = vperm(w1[d], w0[d], 0x05040100u);
+                acc[2 * d + 1][q] = vperm(w1[d], w0[d], 0x07060302u);
+            }
+        }
+    }
+
+    auto load_src = [&](uint32_t (&x)[4], uint32_t t) {
+        const bool real = t < a.k;
+        bload16(x, buf_res(real ? (const uint8_t *)a.src[t] + base : nullptr, real ? nb : 0u), lo);
+    };
+    auto rec = [&](uint32_t idx, uint32_t (&e)[Q]) {
+        if constexpr (RB == 16) {
+            const uint4 v = s_pt[idx];
+            e[0] = v.x; e[1] = v.y; e[2] = v.z; e[3] = v.w;
+        } else if constexpr (RB == 8) {
+            const uint2 v = ((const uint2 *)s_pt)[idx];
+            e[0] = v.x; e[1] = v.y;
+        } else {
+            e[0] = ((const uint32_t *)s_pt)[idx];
+        }
+    };
+    uint32_t xc[4], xn[4];
+    load_src(xc, 0);
+    for (uint32_t t = 0; t < a.k; ++t) {
+        const uint32_t tl = t % STAGE;
+        if (tl == 0) {
+            const uint32_t kn = std::min<uint32_t>(STAGE, a.k - t);
+            const uint4 *g = (const uint4 *)(pt + (size_t)t * 512u * RB);
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < kn * 32u * RB; i += PTAB_BLOCK) s_pt[i] = g[i];
+            __syncthreads();
+        }
+        load_src(xn, t + 1);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const uint32_t x0 = __builtin_amdgcn_ubfe(xc[e >> 1], 16u * (e & 1), 8u);
+            const uint32_t x1 = __builtin_amdgcn_ubfe(xc[e >> 1], 16u * (e & 1) + 8u, 8u);
+            uint32_t r0[Q], r1[Q];
+            rec(tl * 512u + x0, r0);
+            rec(tl * 512u + 256u + x1, r1);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) acc[e][q] = xor3(acc[e][q], r0[q], r1[q]);
+        }
+#pragma unroll
+        for (int d = 0; d < 4; ++d) xc[d] = xn[d];
+    }
+
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        uint32_t w0[4], w1[4];
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            w0[d] = vperm(acc[2 * d + 1][q], acc[2 * d][q], 0x05040100u);
+            w1[d] = vperm(acc[2 * d + 1][q], acc[2 * d][q], 0x07060302u);
+        }
+        const uint32_t j = 2 * q;
+        if (j < mrows) bstore16(buf_res(a.dst[j0 + j] + base, nb), lo, w0);
+        if (j + 1 < mrows) bstore16(buf_res(a.dst[j0 + j + 1] + base, nb), lo, w1);
+    }
+}
+
 // The packed-table launches of one column segment (all pointers + col0
 // 16-byte aligned, ncols a multiple of 16): sources k0..k0+kn of the plan,
 // rows j0..j0+mn (j0 a multiple of 16), groups of R rows.
@@ -466,8 +588,9 @@ hipError_t launch_ptab_segment(const MatrixJob &job, size_t k0, size_t kn, size_
     PtabArgs a{};
     for (size_t t = 0; t < kn; ++t) a.src[t] = job.srcs[k0 + t];
     for (size_t j = 0; j < mn; ++j) a.dst[j] = job.row(j0 + j);
-    a.group_stride = (uint64_t)job.ptab_k * 256u * R;
-    a.ptab = job.ptab + (j0 / R) * a.group_stride + k0 * 256u * R;
+    const uint64_t per_source = (uint64_t)job.nb * 256u * job.nb * R;   // nb tables of nb * R-byte records
+    a.group_stride = (uint64_t)job.ptab_k * per_source;
+    a.ptab = job.ptab + (j0 / R) * a.group_stride + k0 * per_source;
     a.col0 = col0;
     a.ncols = ncols;
     a.k = (uint32_t)kn;
@@ -476,9 +599,15 @@ hipError_t launch_ptab_segment(const MatrixJob &job, size_t k0, size_t kn, size_
     a.ncoltiles = (uint32_t)((ncols + PTAB_BLOCK * 16 - 1) / (PTAB_BLOCK * 16));
     const uint64_t blocks = (uint64_t)a.ncoltiles * a.ngroups;
     using Fn = void (*)(PtabArgs);
-    const Fn fn = R == 16  ? (ow ? (Fn)k_matrix_ptab<16, true> : (Fn)k_matrix_ptab<16, false>)
-                  : R == 8 ? (ow ? (Fn)k_matrix_ptab<8, true> : (Fn)k_matrix_ptab<8, false>)
-                           : (ow ? (Fn)k_matrix_ptab<4, true> : (Fn)k_matrix_ptab<4, false>);
+    Fn fn;
+    if (job.nb == 1)
+        fn = R == 16  ? (ow ? (Fn)k_matrix_ptab<16, true> : (Fn)k_matrix_ptab<16, false>)
+             : R == 8 ? (ow ? (Fn)k_matrix_ptab<8, true> : (Fn)k_matrix_ptab<8, false>)
+                      : (ow ? (Fn)k_matrix_ptab<4, true> : (Fn)k_matrix_ptab<4, false>);
+    else
+        fn = R == 8   ? (ow ? (Fn)k_matrix_ptab2<8, true> : (Fn)k_matrix_ptab2<8, false>)
+             : R == 4 ? (ow ? (Fn)k_matrix_ptab2<4, true> : (Fn)k_matrix_ptab2<4, false>)
+                      : (ow ? (Fn)k_matrix_ptab2<2, true> : (Fn)k_matrix_ptab2<2, false>);
     hipLaunchKernelGGL(fn, dim3((uint32_t)blocks), dim3(PTAB_BLOCK), PTAB_LDS, job.stream, a);
     return hipGetLastError();
 }
@@ -521,7 +650,7 @@ hipError_t launch_matrix(const MatrixJob &job) {
             };
             hipError_t e;
             if ((e = segments(0, head, false)) != hipSuccess) return e;
-            if (job.ptab && nb == 1) {
+            if (job.ptab && (nb == 1 || nb == 2)) {
                 for (uint64_t c = 0; c < body; c += MAX_SEGMENT)
                     if ((e = launch_ptab_segment(job, k0, kn, j0, mn, head + c,
                                                  std::min<uint64_t>(MAX_SEGMENT, body - c), ow)) != hipSuccess)

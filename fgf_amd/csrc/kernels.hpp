@@ -121,12 +121,16 @@ FGF_HD constexpr void coeff_table(int f, uint64_t c, uint32_t *out) {
 // Host-side table builder: writes the entry_dwords(B) dwords of coefficient c.
 void build_coeff_table(int field, uint64_t c, uint32_t *out);
 
-// Packed product tables of a byte-field (k, m) plan: rows in groups of R
-// (4, 8 or 16), and for group g and source t a 256-entry table whose record x
-// (R bytes) holds c(t, g*R + j) * x for j < R (zero past row m). Record x of
-// (g, t) is at byte ((g * k + t) * 256 + x) * R. `vals` are the plan values
-// (coefficient (t, j) at t*m + j); `out` holds packed_table_bytes(k, m, R).
-inline size_t packed_table_bytes(size_t k, size_t m, int R) { return ((m + R - 1) / R) * k * 256 * (size_t)R; }
+// Packed product tables of a (k, m) plan over B-byte elements (B = 1 or 2):
+// rows in groups of R, and for group g, source t and byte position i < B a
+// 256-entry table whose record x (B * R bytes) holds the B-byte products
+// c(t, g*R + j) * (x << 8i) for j < R (zero past row m). Record x of
+// (g, t, i) is at byte (((g * k + t) * B + i) * 256 + x) * B * R. `vals` are
+// the plan values (coefficient (t, j) at t*m + j); `out` holds
+// packed_table_bytes(k, m, R, B).
+inline size_t packed_table_bytes(size_t k, size_t m, int R, int B = 1) {
+    return ((m + R - 1) / R) * k * (size_t)B * 256 * (size_t)B * R;
+}
 void build_packed_tables(int field, const uint64_t *vals, size_t k, size_t m, int R, uint8_t *out);
 
 // Rows per block of the v_perm form: the most the field allows, and the
