@@ -37,20 +37,23 @@ def cauchy_plan_values(k: int, m: int) -> list[int]:
     return [Gf8D.inv(j ^ (m + t)) for t in range(k) for j in range(m)]
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_matrix_rs10_4_encode_traffic.txt")
+# FETCH_SIZE/WRITE_SIZE summaries of the encode, one per plan kernel.
+TRAFFIC_PROFILES = [os.path.join(ROOT, "profiles", n) for n in
+                    ("k_matrix_rs10_4_encode_traffic.txt", "k_matrix_ptab_rs10_4_encode_traffic.txt")]
 
 
 def measured_traffic(shard_bytes: int, kernel: str):
     """HBM bytes per encode launch from the committed FETCH_SIZE/WRITE_SIZE
     profile of this workload (counters scaled by a plain-copy calibration);
     None when the profile does not cover this shard length and kernel."""
-    try:
-        kv = {a.strip(): b.strip() for a, b in (line.split(":", 1) for line in open(TRAFFIC_PROFILE)
-                                                if ":" in line)}
-        if int(kv["shard_bytes"]) == shard_bytes and kv.get("kernel") == kernel:
-            return int(kv["traffic_bytes_per_launch"])
-    except (OSError, KeyError, ValueError):
-        pass
+    for path in TRAFFIC_PROFILES:
+        try:
+            kv = {a.strip(): b.strip() for a, b in (line.split(":", 1) for line in open(path)
+                                                    if ":" in line)}
+            if int(kv["shard_bytes"]) == shard_bytes and kv.get("kernel") == kernel:
+                return int(kv["traffic_bytes_per_launch"])
+        except (OSError, KeyError, ValueError):
+            continue
     return None
 
 
