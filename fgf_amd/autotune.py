@@ -97,15 +97,15 @@ def tune_batch(batch, reps: int = 3, use_cache: bool = True) -> dict:
     # Stage 1: tile shape at a middle launch shape.
     for d in dws:
         for r in rows:
-            cfg = (d, r, 4, 16, 1)
+            cfg = (d, r, 1, 16, 1)
             batch.configure(*cfg)
             trials[cfg] = _time(batch, reps)
     d, r = min(trials, key=trials.get)[:2]
-    # Stage 2: item size and residency for the winning tile. One-tile items
-    # pay an item's fixed cost (claim, descriptor reads, the first load's
-    # full latency) per tile and large items leave a ragged tail, so the
-    # middle sizes are the ones worth timing.
-    for sub in (2, 4, 8):
+    # Stage 2: item size (tiles per wave) and residency for the winning
+    # tile. Small items pay an item's fixed cost (claim, descriptor reads,
+    # the block's hand-over barriers, the first load's full latency) more
+    # often and large items leave a ragged tail.
+    for sub in (1, 2, 4):
         for w in (8, 12, 16):
             cfg = (d, r, sub, w, 1)
             if cfg not in trials:

@@ -48,7 +48,7 @@ class Batch:
         _check(_lib.lib().fgf_batch_create(field.id, ctypes.addressof(arr), n, ctypes.byref(h)))
         self._h = h
 
-    def configure(self, dwords: int = 8, rows_per_item: int = 8, subtiles: int = 4,
+    def configure(self, dwords: int = 8, rows_per_item: int = 8, subtiles: int = 1,
                   waves_per_cu: int = 12, order: int = 1) -> None:
         """Tile/launch shape: see fgf_batch_configure in include/fgf_amd.h."""
         _check(_lib.lib().fgf_batch_configure(self._h, dwords, rows_per_item, subtiles,

@@ -4,11 +4,14 @@
 //
 // The launch reads a device-resident descriptor table: one StripeDesc per
 // stripe and a work list of (stripe, column chunk, row group) items. Every
-// wave pulls items from a device counter with one atomic per item, so uneven
-// stripes balance themselves across the 256 CUs. The counter resets itself:
-// the last wave to retire zeroes it, so a captured hipGraph of the launch
-// replays with no memset node. The per-item body is the same register-tiled
-// perm kernel as the per-call path (matrix_core.hpp).
+// block pulls items from a device counter with one atomic per item, so
+// uneven stripes balance themselves across the 256 CUs, and the four waves
+// of the block share the item: wave w runs its tiles w, w + 4, w + 8, ...,
+// so at any moment the block reads four adjacent tiles of each stream, as
+// the per-call kernel's blocks do. The counter resets itself: the last wave
+// to retire zeroes it, so a captured hipGraph of the launch replays with no
+// memset node. The per-item body is the same register-tiled perm kernel as
+// the per-call path (matrix_core.hpp).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -30,13 +33,17 @@ namespace fgf {
 namespace {
 
 constexpr int BATCH_BLOCK = 256;
+// Waves per block, which share each item's tiles.
+constexpr uint32_t ITEM_WAVES = BATCH_BLOCK / 64;
 
 // Source slots of the per-wave load ring (matrix_run): three loads in flight
 // while one is folded in, fewer for the wide towers whose Karatsuba
 // selectors already hold most of the registers.
 constexpr int ring_slots(int nb) { return nb <= 2 ? 4 : nb == 4 ? 3 : 2; }
 
-// The wave tiles of one item with exactly MC rows, as one pipelined run: a
+// This wave's tiles of one item with exactly MC rows, as one pipelined run:
+// the item's tiles from `tile` on, ITEM_WAVES apart starting at the wave's
+// index (at most subtiles of them; none when the stripe ends first). A
 // compile-time row count leaves the body without per-row guards, so every
 // table load of a source step can be scheduled ahead of the lookups that
 // use it.
@@ -45,16 +52,18 @@ __device__ __forceinline__ void item_tiles(const StripeDesc &s, uint32_t j0, uin
                                            uint32_t subtiles, uint32_t lane) {
     constexpr uint64_t CHUNK = 256u * D;
     const core::GlobalTab<B> tab{s.tab + (size_t)j0 * s.tab_stride_j, s.tab_stride_t, s.tab_stride_j};
-    const uint64_t left = (s.row_len - tile + CHUNK - 1) / CHUNK;
+    tile += (uint64_t)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * CHUNK;
+    const uint64_t left =
+        tile >= s.row_len ? 0u : (s.row_len - tile + ITEM_WAVES * CHUNK - 1) / (ITEM_WAVES * CHUNK);
     const uint32_t nsub = (uint32_t)min<uint64_t>(subtiles, left);
     const core::ConstPtrs<const uint8_t> src{s.srcs};
     const core::ConstPtrs<uint8_t> dst{s.rows + j0};
     if (s.overwrite)
         core::matrix_run<B, MC, D, FP, ring_slots(B), false>(src, dst, tab, s.k, 0, s.row_len, tile,
-                                                             nsub, lane);
+                                                             nsub, lane, ITEM_WAVES);
     else
         core::matrix_run<B, MC, D, FP, ring_slots(B), true>(src, dst, tab, s.k, 0, s.row_len, tile,
-                                                            nsub, lane);
+                                                            nsub, lane, ITEM_WAVES);
 }
 
 template <int B, bool FP, int D, int... Ms>
@@ -78,20 +87,27 @@ __global__ __launch_bounds__(BATCH_BLOCK) __attribute__((amdgpu_waves_per_eu(bat
 void k_batch(const BatchArgs a) {
     const uint32_t lane = threadIdx.x & 63u;
     constexpr uint64_t CHUNK = 256u * D;
-    // The next item is claimed before the current one is processed: the
-    // atomic's latency overlaps with the current item's loads.
+    // The block's first thread claims items and hands them to the block
+    // through LDS. The next item is claimed before the current one is
+    // processed, so the atomic's latency overlaps with the item's loads; the
+    // barriers around the hand-over keep the block's waves on the same item.
+    __shared__ uint32_t s_item;
     uint32_t claim = 0;
-    if (lane == 0) claim = atomicAdd(&a.counter[0], 1u);
-    uint32_t it = __builtin_amdgcn_readfirstlane(claim);
+    if (threadIdx.x == 0) s_item = atomicAdd(&a.counter[0], 1u);
+    __syncthreads();
+    uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
     while (it < a.nitems) {
-        if (lane == 0) claim = atomicAdd(&a.counter[0], 1u);
+        if (threadIdx.x == 0) claim = atomicAdd(&a.counter[0], 1u);
         const WorkItem w = core::ld_const(a.items + it);
         const StripeDesc s = core::ld_const(a.stripes + w.stripe);
         const uint32_t j0 = w.rowgroup * M;
         const uint32_t mrows = __builtin_amdgcn_readfirstlane(min((uint32_t)M, s.m - j0));
-        const uint64_t tile = (uint64_t)w.chunk * a.subtiles * CHUNK;
+        const uint64_t tile = (uint64_t)w.chunk * a.subtiles * ITEM_WAVES * CHUNK;
         item_by_rows<B, FP, D>(mrows, s, j0, tile, a.subtiles, lane, std::make_integer_sequence<int, M>{});
-        it = __builtin_amdgcn_readfirstlane(claim);
+        __syncthreads();  // every wave has read s_item
+        if (threadIdx.x == 0) s_item = claim;
+        __syncthreads();
+        it = __builtin_amdgcn_readfirstlane(s_item);
     }
     // Retire: the last wave out resets the queue for the next launch.
     __threadfence();
@@ -146,13 +162,14 @@ hipError_t launch_batch(int field, int dwords, int rows, const BatchArgs &args, 
 uint32_t batch_block_threads() { return BATCH_BLOCK; }
 
 // Host: the work list for a set of stripes. Items are (stripe, chunk, row
-// group) where a chunk is `subtiles` wave tiles of 256 * dwords bytes;
+// group) where a chunk is subtiles * ITEM_WAVES wave tiles of 256 * dwords
+// bytes (each wave of the block that takes the item runs subtiles of them);
 // `order` 0 keeps stripe order, 1 sorts by descending per-item cost
 // (longest-processing-time first) so the big items start early and the queue
 // drains evenly.
 std::vector<WorkItem> build_work_items(const std::vector<StripeDesc> &stripes, int dwords,
                                        int rows_per_item, int subtiles, int order) {
-    const uint64_t chunk = 256u * (uint64_t)dwords * (uint64_t)subtiles;
+    const uint64_t chunk = 256u * (uint64_t)dwords * (uint64_t)subtiles * ITEM_WAVES;
     const uint32_t rows = (uint32_t)rows_per_item;
     size_t n = 0;
     for (const StripeDesc &d : stripes)
@@ -206,7 +223,7 @@ struct fgf_batch {
     int dwords = 8;
     int order = 1;
     int rows_per_item = 8;
-    int subtiles = 4;
+    int subtiles = 1;
     uint32_t waves_per_cu = 12;
     size_t nstripes = 0;
     std::vector<fgf::StripeDesc> stripes;   // host copy (device pointers inside)
@@ -270,7 +287,7 @@ int batch_build_items(fgf_batch *b) {
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     const uint64_t waves_per_block = fgf::batch_block_threads() / 64;
     const uint64_t want_waves = (uint64_t)cus * b->waves_per_cu;
-    const uint64_t waves = std::max<uint64_t>(1, std::min<uint64_t>(want_waves, b->nitems));
+    const uint64_t waves = std::max<uint64_t>(1, std::min<uint64_t>(want_waves, b->nitems * waves_per_block));
     b->blocks = (uint32_t)((waves + waves_per_block - 1) / waves_per_block);
     batch_release_graph(b);
     return FGF_OK;

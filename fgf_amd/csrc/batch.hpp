@@ -22,7 +22,7 @@ struct StripeDesc {
 
 struct WorkItem {
     uint32_t stripe;
-    uint32_t chunk;      // column chunk of subtiles * 256 * dwords bytes
+    uint32_t chunk;      // column chunk of subtiles * 4 wave tiles of 256 * dwords bytes
     uint32_t rowgroup;   // rows [rowgroup * M, rowgroup * M + M)
     uint32_t pad;
 };

@@ -663,9 +663,9 @@ __device__ __forceinline__ void matrix_tile_staged(SrcPtrs src, DstPtrs dst, voi
     }
 }
 
-// A run of nsub consecutive wave tiles of one operation (tile0, tile0 +
-// 256*D, ...), rows[j] (=|^=) sum_t c(t, j) * src[t], as ONE software
-// pipeline. Each tile is a sequence of steps: the k sources, then (when
+// A run of nsub wave tiles of one operation, tstride tiles apart (tile0,
+// tile0 + tstride*256*D, ...), rows[j] (=|^=) sum_t c(t, j) * src[t], as ONE
+// software pipeline. Each tile is a sequence of steps: the k sources, then (when
 // accumulating) the M destination rows, each step one D-dword stream load
 // that lands in a ring of R register slots. R - 1 loads are in flight while
 // one slot is folded in, and the ring runs straight across tile boundaries,
@@ -679,7 +679,7 @@ __device__ __forceinline__ void matrix_tile_staged(SrcPtrs src, DstPtrs dst, voi
 template <int B, int M, int D, bool FP, int R, bool ACC, typename SrcPtrs, typename DstPtrs, typename Tab>
 __device__ __forceinline__ void matrix_run(SrcPtrs src, DstPtrs dst, const Tab &tab, uint32_t k,
                                            uint64_t col0, uint64_t ncols, uint64_t tile0,
-                                           uint32_t nsub, uint32_t lane) {
+                                           uint32_t nsub, uint32_t lane, uint32_t tstride = 1) {
     constexpr int P = D / 4;
     constexpr int G = D / B;
     constexpr uint64_t CHUNK = 256u * D;
@@ -691,19 +691,20 @@ __device__ __forceinline__ void matrix_run(SrcPtrs src, DstPtrs dst, const Tab &
 #pragma unroll
     for (int i = 0; i < P; ++i) lo[i] = (uint32_t)i * 1024u + lane * 16u;
 
-    // Bytes before ncols of tile t0 + i, in 32-bit wave-uniform arithmetic:
-    // whole tiles, then the tail tile, then nothing.
+    // Bytes before ncols of run tile i (tile t0 + i * tstride), in 32-bit
+    // wave-uniform arithmetic: whole tiles, then the tail tile, then nothing.
     const uint32_t t0 = (uint32_t)(tile0 / CHUNK), nfull = (uint32_t)(ncols / CHUNK);
     const uint32_t tail = (uint32_t)(ncols % CHUNK);
     auto tile_bytes = [&](uint32_t i) -> uint32_t {
-        return t0 + i < nfull ? (uint32_t)CHUNK : t0 + i == nfull ? tail : 0u;
+        const uint32_t t = t0 + i * tstride;
+        return t < nfull ? (uint32_t)CHUNK : t == nfull ? tail : 0u;
     };
 
     // Issue cursor: step, tile, step within the tile. A step past the end
     // loads through an empty resource (zeros, no memory access).
     uint32_t is = 0, ic = 0, it = 0;
     auto issue = [&](uint32_t (&x)[D]) {
-        const uint64_t base = tile0 + (uint64_t)ic * CHUNK;
+        const uint64_t base = tile0 + (uint64_t)ic * tstride * CHUNK;
         const uint8_t *p = (it < k ? (const uint8_t *)src[it] : (const uint8_t *)dst[it - k]) + col0 + base;
         const BufRes r = buf_res(p, is < nsteps ? tile_bytes(ic) : 0u);
 #pragma unroll
@@ -764,7 +765,7 @@ __device__ __forceinline__ void matrix_run(SrcPtrs src, DstPtrs dst, const Tab &
             issue(ring[r]);
             ++cs;
             if (real && ++ct == per) {
-                const uint64_t base = tile0 + (uint64_t)cc * CHUNK;
+                const uint64_t base = tile0 + (uint64_t)cc * tstride * CHUNK;
                 const uint32_t n = tile_bytes(cc);
 #pragma unroll
                 for (int j = 0; j < M; ++j) {

@@ -339,9 +339,9 @@ int launch_slot(fgf_server *s, Slot &sl) {
         nptrs += plan->rows + plan->cols;
     }
     // Item size: a launch too small to give every resident wave one wave
-    // tile uses items of a single tile, spreading a lone request over as many
-    // waves as it has tiles (its latency is one wave's serial work); larger
-    // launches use the configured multi-tile items.
+    // tile uses items of one tile per wave, spreading a lone request over as
+    // many waves as it has tiles (its latency is one wave's serial work);
+    // larger launches use the configured multi-tile items.
     const uint64_t tile_bytes = 256u * (uint64_t)dwords;
     uint64_t tiles = 0;
     for (const StripeDesc &d : stripes)
@@ -378,7 +378,7 @@ int launch_slot(fgf_server *s, Slot &sl) {
         return sdev("descriptor upload", e);
     const uint32_t wpb = batch_block_threads() / 64;
     const uint64_t want = (uint64_t)s->cus * s->cfg.waves_per_cu;
-    const uint64_t waves = std::max<uint64_t>(1, std::min<uint64_t>(want, items.size()));
+    const uint64_t waves = std::max<uint64_t>(1, std::min<uint64_t>(want, items.size() * wpb));
     const uint32_t blocks = (uint32_t)((waves + wpb - 1) / wpb);
     BatchArgs a{(const StripeDesc *)(sl.d_blob + off_str), (const WorkItem *)(sl.d_blob + off_items),
                 (uint32_t)items.size(), blocks * wpb, (uint32_t)subtiles, sl.d_counter};

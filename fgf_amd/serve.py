@@ -60,7 +60,7 @@ def _c_request(r: Request, keep: list) -> _Request:
 class Server:
     def __init__(self, hbm_budget_bytes: int = 64 << 30, slots: int = 4,
                  max_batch_stripes: int = 512, plan_cache_entries: int = 256, dwords: int = 8,
-                 rows_per_item: int = 8, subtiles: int = 4, waves_per_cu: int = 8):
+                 rows_per_item: int = 8, subtiles: int = 1, waves_per_cu: int = 8):
         cfg = _Config(hbm_budget_bytes, slots, max_batch_stripes, plan_cache_entries, dwords,
                       rows_per_item, subtiles, waves_per_cu)
         h = ctypes.c_void_p()

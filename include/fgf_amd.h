@@ -327,9 +327,11 @@ int fgf_batch_create(fgf_field field, const fgf_stripe *stripes, size_t nstripes
 /* Tile and launch shape (the autotuner's knobs): dwords per lane per source
  * (4 or 8; 8 for 4- and 8-byte elements), destination rows per wave (4 or 8
  * for 1- and 2-byte elements, 4 for GF(2^32), 2 for GF(2^64)), wave tiles per
- * work item (1..64), resident waves per CU (1..32), and work order
+ * wave per work item (1..64; a work item is that many tiles for each of the
+ * four waves of the block that takes it, interleaved so the block reads
+ * adjacent tiles together), resident waves per CU (1..32), and work order
  * (0 = stripe order, 1 = largest items first). A new batch starts at 8
- * dwords, the field's most rows, 4 subtiles, 12 waves per CU, order 1. */
+ * dwords, the field's most rows, 1 subtile, 12 waves per CU, order 1. */
 int fgf_batch_configure(fgf_batch *batch, int dwords, int rows_per_item, int subtiles,
                         int waves_per_cu, int order);
 /* One persistent launch over every stripe, on `stream`. */
