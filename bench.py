@@ -341,7 +341,7 @@ def bench_batched(n_stripes: int = 4096, seed: int = 4096, reps: int = 5) -> dic
 
 
 def profile_batched(launches: int, n_stripes: int = 4096, seed: int = 4096,
-                    config=(8, 8, 1, 12, 1)) -> dict:
+                    config=(8, 8, 2, 12, 0)) -> dict:
     """Only the batched launch of the uneven-stripe mix, `launches` times, at
     a fixed tile config (dwords, rows per item, subtiles, waves per CU,
     order): the run that the batch kernel's rocprofv3 counter and trace
