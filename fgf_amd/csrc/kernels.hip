@@ -242,6 +242,36 @@ size_t k_chunk(int nb) {
 // 32 bytes so a lane holds whole 4-element groups.
 constexpr int dwords_for(int nb, int m) { return nb == 1 ? 4 : 8; }
 
+// Launches of byte fields at up to SMALL_MAX_ROWS rows per group whose 1 KiB
+// wave tiles would give the GPU fewer than SMALL_WAVES waves take 512- or
+// 256-byte tiles instead (2 or 1 dwords per lane), the widest that reaches
+// SMALL_WAVES: a cache-resident shape then runs several waves per SIMD.
+constexpr int SMALL_MAX_ROWS = 8;
+constexpr uint64_t SMALL_WAVES = 2048;
+int small_dwords(uint64_t ncols, uint64_t ngroups) {
+    for (int d = 4; d > 1; d /= 2)
+        if ((ncols + 256u * d - 1) / (256u * d) * ngroups >= SMALL_WAVES) return d;
+    return 1;
+}
+
+template <int M, int D>
+MatrixFn pick_small_m(bool ow) {
+    return ow ? (MatrixFn)k_matrix<1, false, M, D, true> : (MatrixFn)k_matrix<1, false, M, D, false>;
+}
+
+template <int D, int... Ms>
+MatrixFn pick_small_from(int m, bool ow, std::integer_sequence<int, Ms...>) {
+    MatrixFn fn = nullptr;
+    ((m == Ms + 1 ? (fn = pick_small_m<Ms + 1, D>(ow), 0) : 0), ...);
+    return fn;
+}
+
+// Byte-field kernel with D = 1 or 2 dwords per lane, m <= SMALL_MAX_ROWS.
+MatrixFn pick_small(int m, bool ow, int d) {
+    return d == 2 ? pick_small_from<2>(m, ow, std::make_integer_sequence<int, SMALL_MAX_ROWS>{})
+                  : pick_small_from<1>(m, ow, std::make_integer_sequence<int, SMALL_MAX_ROWS>{});
+}
+
 template <int B, bool FP, int M>
 MatrixFn pick_m(bool ow) {
     constexpr int D = dwords_for(B, M);
@@ -323,16 +353,16 @@ hipError_t launch_segment(const MatrixJob &job, MatrixArgs a, size_t k0, size_t 
             mn = full;
             a.m = (uint32_t)mn;
         }
-        const int d = dwords_for(nb, mt);
-        const uint64_t block_bytes = (uint64_t)BLOCK / 64 * 256u * d;
         a.ngroups = (uint32_t)((mn + mt - 1) / mt);
+        const int d = nb == 1 && mt <= SMALL_MAX_ROWS ? small_dwords(ncols, a.ngroups) : dwords_for(nb, mt);
+        const uint64_t block_bytes = (uint64_t)BLOCK / 64 * 256u * d;
         a.ncoltiles = (uint32_t)((ncols + block_bytes - 1) / block_bytes);
         const uint64_t blocks = a.ngroups == 1
                                     ? a.ncoltiles
                                     : (uint64_t)((a.ncoltiles + 7) / 8) * 8u * a.ngroups;
         const size_t lds = (nb == 8 ? std::min<size_t>(kn, stage8(mt)) : kn) * (size_t)mt * lds_record_bytes(nb, mt);
-        hipLaunchKernelGGL(pick_vector(nb, job.fan_paar, mt, ow), dim3((uint32_t)blocks), dim3(BLOCK), lds,
-                           job.stream, a);
+        hipLaunchKernelGGL(d < 4 ? pick_small(mt, ow, d) : pick_vector(nb, job.fan_paar, mt, ow),
+                           dim3((uint32_t)blocks), dim3(BLOCK), lds, job.stream, a);
     } else {
         a.ngroups = 1;
         const uint64_t groups = (ncols / nb + 3) / 4;

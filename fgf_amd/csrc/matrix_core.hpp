@@ -250,6 +250,7 @@ __device__ __forceinline__ void kara_accumulate(uint32_t *acc, const uint32_t *l
 // this lets the 72/28 read/write mix of an encode run at about 6 TB/s where
 // cached accesses to the same addresses saturate near 5.4 TB/s.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 // Every stream lives in device (global) memory; saying so lets pointers that
 // were loaded from memory use global_load/store with a scalar base instead
@@ -292,6 +293,40 @@ __device__ __forceinline__ void bload16(uint32_t *x, BufRes r, uint32_t off) {
 __device__ __forceinline__ void bstore16(BufRes r, uint32_t off, const uint32_t *w) {
     const u32x4 v = {w[0], w[1], w[2], w[3]};
     __builtin_amdgcn_raw_buffer_store_b128(v, r, off, 0, BUF_NT);
+}
+
+// One lane's D dwords of a wave tile of 256 * D bytes: D / 4 accesses of 16
+// bytes, 1 KiB apart (lane * 16 within each), or for D = 1 and 2 one access
+// of D dwords at lane * 4D. Offsets are relative to the tile.
+template <int D>
+__device__ __forceinline__ uint32_t lane_off(uint32_t lane, int i) {
+    return D >= 4 ? (uint32_t)i * 1024u + lane * 16u : lane * (4u * D);
+}
+
+template <int D>
+__device__ __forceinline__ void bload_lane(uint32_t *x, BufRes r, uint32_t lane) {
+    if constexpr (D >= 4) {
+#pragma unroll
+        for (int i = 0; i < D / 4; ++i) bload16(&x[4 * i], r, lane_off<D>(lane, i));
+    } else if constexpr (D == 2) {
+        const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, lane_off<D>(lane, 0), 0, BUF_NT);
+        x[0] = v.x; x[1] = v.y;
+    } else {
+        x[0] = __builtin_amdgcn_raw_buffer_load_b32(r, lane_off<D>(lane, 0), 0, BUF_NT);
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void bstore_lane(BufRes r, uint32_t lane, const uint32_t *w) {
+    if constexpr (D >= 4) {
+#pragma unroll
+        for (int i = 0; i < D / 4; ++i) bstore16(r, lane_off<D>(lane, i), &w[4 * i]);
+    } else if constexpr (D == 2) {
+        const u32x2 v = {w[0], w[1]};
+        __builtin_amdgcn_raw_buffer_store_b64(v, r, lane_off<D>(lane, 0), 0, BUF_NT);
+    } else {
+        __builtin_amdgcn_raw_buffer_store_b32(w[0], r, lane_off<D>(lane, 0), 0, BUF_NT);
+    }
 }
 
 
@@ -488,22 +523,19 @@ template <int B, int M, int D, bool FP = false, typename SrcPtrs, typename DstPt
 __device__ __forceinline__ void matrix_tile(SrcPtrs src, DstPtrs dst, const Tab &tab, uint32_t k,
                                             bool ow, uint64_t col0, uint64_t ncols,
                                             uint64_t tile, uint32_t lane) {
-    constexpr int P = D / 4;
     constexpr int G = D / B;
     constexpr uint32_t CHUNK = 256u * D;
-    static_assert(D % 4 == 0 && D % B == 0, "tile shape");
+    // Byte fields also take 1- and 2-dword lanes (wave tiles of 256 and 512
+    // bytes) for launches too small to fill the GPU at 4.
+    static_assert((D % 4 == 0 || (B == 1 && (D == 1 || D == 2))) && D % B == 0, "tile shape");
 
-    uint32_t lo[P];
-#pragma unroll
-    for (int i = 0; i < P; ++i) lo[i] = (uint32_t)i * 1024u + lane * 16u;
     const uint32_t nb = ncols - tile < CHUNK ? (uint32_t)(ncols - tile) : CHUNK;
     const uint64_t base = col0 + tile;
 
     auto load_src = [&](uint32_t (&x)[D], uint32_t t) {
         const bool real = t < k;
         const BufRes r = buf_res((const uint8_t *)src[real ? t : 0u] + base, real ? nb : 0u);
-#pragma unroll
-        for (int i = 0; i < P; ++i) bload16(&x[4 * i], r, lo[i]);
+        bload_lane<D>(x, r, lane);
     };
 
     uint32_t acc[M][D];
@@ -514,9 +546,7 @@ __device__ __forceinline__ void matrix_tile(SrcPtrs src, DstPtrs dst, const Tab 
 #pragma unroll
             for (int d = 0; d < D; ++d) w[d] = 0;
         } else {
-            const BufRes r = buf_res((const uint8_t *)dst[j] + base, nb);
-#pragma unroll
-            for (int i = 0; i < P; ++i) bload16(&w[4 * i], r, lo[i]);
+            bload_lane<D>(w, buf_res((const uint8_t *)dst[j] + base, nb), lane);
         }
 #pragma unroll
         for (int g = 0; g < G; ++g) to_planes<B>(&w[g * B], &acc[j][g * B]);
@@ -541,9 +571,7 @@ __device__ __forceinline__ void matrix_tile(SrcPtrs src, DstPtrs dst, const Tab 
         uint32_t w[D];
 #pragma unroll
         for (int g = 0; g < G; ++g) from_planes<B>(&acc[j][g * B], &w[g * B]);
-        const BufRes r = buf_res((uint8_t *)dst[j] + base, nb);
-#pragma unroll
-        for (int i = 0; i < P; ++i) bstore16(r, lo[i], &w[4 * i]);
+        bstore_lane<D>(buf_res((uint8_t *)dst[j] + base, nb), lane, w);
     }
 }
 

@@ -232,6 +232,34 @@ def test_matrix_every_source_count_mod_4_over_ragged_tiles(f):
             assert np.array_equal(host(acc), want), (row_len, k, "accumulate")
 
 
+@pytest.mark.parametrize("f", [Gf8D, Gf8B, FanPaar8], ids=lambda f: NAMES[f.id])
+def test_matrix_lane_widths_of_small_launches(f):
+    # Byte-field launches of up to 8 rows per group that would give the GPU
+    # fewer than 2048 waves at 16 bytes per lane take 8 or 4 bytes per lane
+    # (wave tiles of 512 and 256 bytes). These row lengths take 4-, 8- and
+    # 16-byte lanes in turn, each with a ragged last tile (1 MiB + 48: 2049
+    # waves of 512 bytes; 2 MiB + 16: 2049 of 1 KiB), and 9 rows take the
+    # 16-byte lanes at any length; overwrite and accumulate.
+    name = NAMES[f.id]
+    k = 6
+    for row_len in (256 * 5 + 48, (1 << 20) + 48, (2 << 20) + 16):
+        for nrows in ((1, 4, 8, 9) if row_len < (1 << 20) else (4, 9) if row_len < (2 << 20) else (8,)):
+            seed = row_len % 977 + 31 * nrows
+            srcs = [noise(row_len, 0xD00 + seed + t) for t in range(k)]
+            coeffs = rand_coeffs(f, k * nrows, 0xE00 + seed)
+            plan = ops.Plan.matrix(f, k, nrows, coeffs)
+            d_srcs = [dev(x) for x in srcs]
+            got = dev(noise(row_len * nrows, 0xF00 + seed))
+            ops.dot_product_matrix_with(f, got, row_len, nrows, plan, d_srcs)
+            want = oracle_matrix(name, row_len, nrows, coeffs, srcs)
+            assert np.array_equal(host(got), want), (row_len, nrows, "overwrite")
+            init = noise(row_len * nrows, 0x1000 + seed)
+            acc = dev(init)
+            ops.mul_add_matrix_with(f, acc, row_len, nrows, plan, d_srcs)
+            want = oracle_matrix(name, row_len, nrows, coeffs, srcs, init)
+            assert np.array_equal(host(acc), want), (row_len, nrows, "accumulate")
+
+
 @pytest.mark.parametrize("f", [Gf8D, Gf32, FanPaar64], ids=lambda f: NAMES[f.id])
 def test_host_buffer_form_matches_device_form(f):
     # The host form pipelines 8 MiB column chunks through three streams and
