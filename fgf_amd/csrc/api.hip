@@ -164,6 +164,7 @@ struct Tables {
     const uint32_t *host = nullptr;
     size_t stride_t = 0, stride_j = 0;
     const fgf_plan *plan = nullptr;   // a matrix plan's kernel choice applies
+    int device = -1;                  // a plan's device (its tables live there)
 };
 
 Tables plan_tables(const fgf_plan *p, size_t stride_t, size_t stride_j, size_t first = 0) {
@@ -171,6 +172,7 @@ Tables plan_tables(const fgf_plan *p, size_t stride_t, size_t stride_j, size_t f
     t.dev = p->d_tab + first * entry_dw(p->nb);
     t.stride_t = stride_t;
     t.stride_j = stride_j;
+    t.device = p->device;
     return t;
 }
 
@@ -217,6 +219,12 @@ Rows rows_at(uint8_t *base, const size_t *offsets, size_t m) {
 
 int run_matrix(int f, const Tables &tb, const Rows &rows, const uint8_t *const *srcs, size_t k,
                size_t row_len, bool overwrite, hipStream_t s) {
+    // A plan's tables are device memory of the device it was created on.
+    if (tb.device >= 0) {
+        int cur = 0;
+        if (hipGetDevice(&cur) == hipSuccess && cur != tb.device)
+            return fail("Plan: created on device %d, used on device %d", tb.device, cur);
+    }
     MatrixJob job;
     job.nb = field_bytes(f);
     job.fan_paar = is_fan_paar(f);

@@ -344,11 +344,16 @@ int fgf_batch_create(fgf_field f, const fgf_stripe *stripes, size_t n, fgf_batch
     if (!valid_field(f)) return bfail("unknown field id %d", (int)f);
     if (!out) return bfail("fgf_batch_create: output pointer is null");
     const int nb = field_bytes(f);
+    int dev = 0;
+    (void)hipGetDevice(&dev);
     size_t nptrs = 0;
     for (size_t s = 0; s < n; ++s) {
         const fgf_stripe &st = stripes[s];
         const fgf_plan *p = st.plan;
         if (!p) return bfail("batch stripe %zu: plan is null", s);
+        if (p->device != dev)
+            return bfail("batch stripe %zu: plan was created on device %d, the batch is on device %d", s,
+                         p->device, dev);
         if (p->field != f)
             return bfail("batch stripe %zu: plan is over %s, not %s", s, field_name(p->field), field_name(f));
         if (p->rows == 0 || p->cols == 0)
