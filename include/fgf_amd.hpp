@@ -142,6 +142,18 @@ class Plan {
         return matrix(d.first, d.second, values());
     }
 
+    /* Kernel choice of the per-call matrix launches (no counterpart in the
+     * reference; fgf_plan_set_kernel): form 0 v_perm tables, 1 packed
+     * product tables (1- and 2-byte fields); rows_per_group 0 = default. */
+    void set_kernel(int form, int rows_per_group = 0) {
+        detail::check(fgf_plan_set_kernel(h_, form, rows_per_group));
+    }
+    std::pair<int, int> kernel() const {
+        int form = 0, rows = 0;
+        detail::check(fgf_plan_get_kernel(h_, &form, &rows));
+        return {form, rows};
+    }
+
     std::size_t len() const { return fgf_plan_len(h_); }
     bool is_empty() const { return len() == 0; }
     std::pair<std::size_t, std::size_t> dimensions() const {

@@ -315,12 +315,18 @@ void check_dot_product_matrix(uint64_t seed) {
             Vec init = noise(row_len * nrows + extra, seed + 99);
             Dev d(init), e(init);
             ops::dot_product_matrix<F>(d.b(), row_len, nrows, terms);
-            const auto plan = Plan<F>::matrix(k, nrows, flat);
+            auto plan = Plan<F>::matrix(k, nrows, flat);
             ops::dot_product_matrix_with<F>(e.b(), row_len, nrows, plan, views);
             Vec want = oracle_matrix<F>(row_len, nrows, flat, srcs);
             want.insert(want.end(), init.end() - extra, init.end());
             REQUIRE(d.get() == want, "terms form, row_len %zu nrows %zu", row_len, nrows);
             REQUIRE(e.get() == want, "plan form, row_len %zu nrows %zu", row_len, nrows);
+            // The packed product-table kernel of the same plan.
+            plan.set_kernel(1);
+            REQUIRE(plan.kernel() == std::make_pair(1, 0), "kernel choice not kept");
+            Dev p(init);
+            ops::dot_product_matrix_with<F>(p.b(), row_len, nrows, plan, views);
+            REQUIRE(p.get() == want, "packed plan form, row_len %zu nrows %zu", row_len, nrows);
         }
     }
 }
