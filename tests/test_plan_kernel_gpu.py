@@ -33,7 +33,9 @@ def oracle(f, L, m, vals, srcs, init=None):
 
 
 SHAPES = [(1, 1), (3, 5), (10, 4), (17, 9), (5, 8), (5, 16), (64, 16), (70, 20), (33, 3)]
-LENGTHS = [16, 1040, 3 * 4096 + 48, 1000]      # 1000: a tail past the last 16 bytes
+# 1000: a tail past the last 16 bytes; 5 * 8192 + 1040: several packed-table
+# blocks (8 KiB of columns each) and a ragged last one.
+LENGTHS = [16, 1040, 3 * 4096 + 48, 1000, 5 * 8192 + 1040]
 
 
 @pytest.mark.parametrize("f", [Gf8B, Gf8D, FanPaar8, Gf16, FanPaar16], ids=lambda f: NAMES[f.id])
