@@ -262,7 +262,7 @@ def _cpu_model() -> str:
     return "unknown"
 
 
-def bench_batched(n_stripes: int = 4096, seed: int = 4096, reps: int = 5) -> dict:
+def bench_batched(n_stripes: int = 4096, seed: int = 4096, reps: int = 15) -> dict:
     """The uneven-stripe mix in one persistent launch vs one launch per stripe.
 
     Stripes: k ~ U[4, 32], m ~ U[2, 8], shard length log-uniform on
@@ -295,16 +295,17 @@ def bench_batched(n_stripes: int = 4096, seed: int = 4096, reps: int = 5) -> dic
     def timed(fn):
         # A pre-roll first: after the host-bound per-call leg (or idle time)
         # the GPU's clocks take some tens of ms of load to come back up.
+        # Then back-to-back launches with an event between each pair (no
+        # host sync between them), the median of the per-launch times.
         _preroll(fn, 100.0)
-        times = []
-        for _ in range(reps):
-            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            a.record(stream)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+        ev[0].record(stream)
+        for i in range(reps):
             fn()
-            b.record(stream)
-            torch.cuda.synchronize()
-            times.append(a.elapsed_time(b))
-        return sorted(times)[len(times) // 2]
+            ev[i + 1].record(stream)
+        torch.cuda.synchronize()
+        times = sorted(ev[i].elapsed_time(ev[i + 1]) for i in range(reps))
+        return times[len(times) // 2]
 
     def outputs():
         return [torch.cat(workloads.stripe_buffers(pool, sp)[1]).clone() for sp in specs]

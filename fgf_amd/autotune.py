@@ -67,19 +67,18 @@ def candidates(nb: int):
 
 
 def _time(batch, reps: int) -> float:
+    """Median of `reps` back-to-back launches (an event between each pair,
+    no host sync between them) after one untimed launch."""
     import torch
     s = torch.cuda.current_stream()
     batch.run()
-    torch.cuda.synchronize()
-    out = []
-    for _ in range(reps):
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record(s)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+    ev[0].record(s)
+    for i in range(reps):
         batch.run()
-        b.record(s)
-        torch.cuda.synchronize()
-        out.append(a.elapsed_time(b))
-    return median(out)
+        ev[i + 1].record(s)
+    torch.cuda.synchronize()
+    return median(ev[i].elapsed_time(ev[i + 1]) for i in range(reps))
 
 
 def tune_batch(batch, reps: int = 3, use_cache: bool = True) -> dict:

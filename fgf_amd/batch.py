@@ -14,7 +14,7 @@ from typing import Sequence
 
 from . import _lib
 from .field import Field, _check
-from .ops import Plan, _buf, _stream
+from .ops import Plan, _buf, _current_stream
 
 
 class _CStripe(ctypes.Structure):
@@ -47,6 +47,11 @@ class Batch:
         h = ctypes.c_void_p()
         _check(_lib.lib().fgf_batch_create(field.id, ctypes.addressof(arr), n, ctypes.byref(h)))
         self._h = h
+        # The device whose current stream a call without one runs on (every
+        # buffer of a batch is on one device): found once here, so a call
+        # costs one stream query, not a scan of the stripes.
+        self._dev = next((b.get_device() for s in self.stripes for b in s.rows
+                          if getattr(b, "is_cuda", False)), None)
 
     def configure(self, dwords: int = 8, rows_per_item: int = 8, subtiles: int = 1,
                   waves_per_cu: int = 12, order: int = 1) -> None:
@@ -58,8 +63,7 @@ class Batch:
     def _s(self, stream):
         if stream is not None:
             return int(stream)
-        bufs = [b for s in self.stripes for b in s.rows[:1]]
-        return _stream(None, *bufs)
+        return 0 if self._dev is None else _current_stream(self._dev)
 
     def run(self, stream=None) -> None:
         _check(_lib.lib().fgf_batch_run(self._h, self._s(stream)))

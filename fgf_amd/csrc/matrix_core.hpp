@@ -280,13 +280,21 @@ typedef __amdgpu_buffer_rsrc_t BufRes;
 // (a zero format would fail the range check of every access).
 constexpr int BUF_RES_DWORD3 = 0x00020000;
 constexpr int BUF_NT = 2;  // cache-policy operand: non-temporal
+// Source loads of 4- and 8-byte towers keep the default policy: with m
+// above the rows of one group, the blocks of the other row groups (dealt to
+// the same XCD right after, see map_block) read the same source tiles, and
+// a non-temporal line is dropped from L2 before they come. Those kernels
+// are bound by VALU and LDS, not HBM, so a streaming hint gains nothing.
+template <int B>
+constexpr int src_policy() { return B >= 4 ? 0 : BUF_NT; }
 
 __device__ __forceinline__ BufRes buf_res(const void *p, uint32_t bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, BUF_RES_DWORD3);
 }
 
+template <int POL = BUF_NT>
 __device__ __forceinline__ void bload16(uint32_t *x, BufRes r, uint32_t off) {
-    const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, BUF_NT);
+    const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, POL);
     x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
 }
 
@@ -303,16 +311,16 @@ __device__ __forceinline__ uint32_t lane_off(uint32_t lane, int i) {
     return D >= 4 ? (uint32_t)i * 1024u + lane * 16u : lane * (4u * D);
 }
 
-template <int D>
+template <int D, int POL = BUF_NT>
 __device__ __forceinline__ void bload_lane(uint32_t *x, BufRes r, uint32_t lane) {
     if constexpr (D >= 4) {
 #pragma unroll
-        for (int i = 0; i < D / 4; ++i) bload16(&x[4 * i], r, lane_off<D>(lane, i));
+        for (int i = 0; i < D / 4; ++i) bload16<POL>(&x[4 * i], r, lane_off<D>(lane, i));
     } else if constexpr (D == 2) {
-        const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, lane_off<D>(lane, 0), 0, BUF_NT);
+        const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, lane_off<D>(lane, 0), 0, POL);
         x[0] = v.x; x[1] = v.y;
     } else {
-        x[0] = __builtin_amdgcn_raw_buffer_load_b32(r, lane_off<D>(lane, 0), 0, BUF_NT);
+        x[0] = __builtin_amdgcn_raw_buffer_load_b32(r, lane_off<D>(lane, 0), 0, POL);
     }
 }
 
@@ -535,7 +543,7 @@ __device__ __forceinline__ void matrix_tile(SrcPtrs src, DstPtrs dst, const Tab 
     auto load_src = [&](uint32_t (&x)[D], uint32_t t) {
         const bool real = t < k;
         const BufRes r = buf_res((const uint8_t *)src[real ? t : 0u] + base, real ? nb : 0u);
-        bload_lane<D>(x, r, lane);
+        bload_lane<D, src_policy<B>()>(x, r, lane);
     };
 
     uint32_t acc[M][D];
@@ -640,7 +648,7 @@ __device__ __forceinline__ void matrix_tile_staged(SrcPtrs src, DstPtrs dst, voi
         const bool real = t < k;
         const BufRes r = buf_res((const uint8_t *)src[real ? t : 0u] + base, real ? nb : 0u);
 #pragma unroll
-        for (int i = 0; i < P; ++i) bload16(&x[4 * i], r, lo[i]);
+        for (int i = 0; i < P; ++i) bload16<src_policy<B>()>(&x[4 * i], r, lo[i]);
     };
 
     uint32_t acc[M][D];
