@@ -456,6 +456,8 @@ def bench_configs(seed: int = 31) -> dict:
       data and two parity shards rebuilt in one pass (checked by round trip);
     * k = 32, m = 8, 16 MiB per source for every tower field, uniform
       nonzero coefficients (VALU-bound: m*B byte products per source byte);
+    * the single-row ops (mul_add, mul_into, mul_assign, add_assign) over
+      256 MiB rows;
     * mul_elementwise on 2^26 uniform Gf64 pairs.
     """
     import numpy as np
@@ -565,6 +567,41 @@ def bench_configs(seed: int = 31) -> dict:
                          byte_products_per_source_byte=m * f.BYTES, kernel=_kernel_name(plan),
                          autotune=tuned))
     del src, rows
+
+    # Single-row ops over 256 MiB rows (ops.rs:323-450): accumulate reads the
+    # source and the destination and writes the destination (3 L), the
+    # overwriting forms move 2 L. Each is checked on a slice after its first
+    # launch.
+    L = 256 << 20
+    src, dst = rnd(L), rnd(L)
+
+    def elems(f, t):
+        raw = t.cpu().numpy().tobytes()
+        return [int.from_bytes(raw[i:i + f.BYTES], "little") for i in range(0, len(raw), f.BYTES)]
+
+    def packed(f, xs):
+        return b"".join(x.to_bytes(f.BYTES, "little") for x in xs)
+
+    single = [
+        ("mul_add", Gf8D, 0x53, 3, lambda f, c: ops.mul_add(f, dst, c, src),
+         lambda f, c, d0, s0: [d ^ f.mul(c, s) for d, s in zip(d0, s0)]),
+        ("mul_into", Gf8D, 0x53, 2, lambda f, c: ops.mul_into(f, dst, c, src),
+         lambda f, c, d0, s0: [f.mul(c, s) for s in s0]),
+        ("mul_assign", Gf8D, 0x53, 2, lambda f, c: ops.mul_assign(f, dst, c),
+         lambda f, c, d0, s0: [f.mul(c, d) for d in d0]),
+        ("add_assign", Gf8D, 1, 3, lambda f, c: ops.add_assign(f, dst, src),
+         lambda f, c, d0, s0: [d ^ s for d, s in zip(d0, s0)]),
+        ("mul_add", Gf16, 0xBEEF, 3, lambda f, c: ops.mul_add(f, dst, c, src),
+         lambda f, c, d0, s0: [d ^ f.mul(c, s) for d, s in zip(d0, s0)]),
+        ("mul_add", Gf64, 0x0123456789ABCDEF, 3, lambda f, c: ops.mul_add(f, dst, c, src),
+         lambda f, c, d0, s0: [d ^ f.mul(c, s) for d, s in zip(d0, s0)]),
+    ]
+    for op, f, c, passes, call, host in single:
+        d0, s0 = elems(f, dst[:probe]), elems(f, src[:probe])
+        call(f, c)
+        assert dst[:probe].cpu().numpy().tobytes() == packed(f, host(f, c, d0, s0)), (op, f.NAME)
+        legs.append(_leg(f"{f.NAME} {op}, 256 MiB row", _median_launch_ms(lambda: call(f, c)), passes * L))
+    del src, dst
 
     # Gf64 elementwise, 2^26 pairs.
     n = (1 << 26) * 8

@@ -272,6 +272,15 @@ MatrixFn pick_small(int m, bool ow, int d) {
                   : pick_small_from<1>(m, ow, std::make_integer_sequence<int, SMALL_MAX_ROWS>{});
 }
 
+// One overwritten byte row from at most WIDE_ROW_MAX_K sources (mul_into,
+// mul_assign, a one- or two-term dot_product) streams one read and one write
+// per wave tile and does almost no arithmetic: 2 KiB wave tiles (8 dwords
+// per lane) keep twice the bytes in flight and move 0.70-0.71 of HBM peak
+// against 0.63-0.64 with 1 KiB tiles (256 MiB rows). Accumulating rows read
+// two streams per tile already and stay at 1 KiB (0.79 either way).
+constexpr uint32_t WIDE_ROW_MAX_K = 2;
+constexpr int WIDE_ROW_DWORDS = 8;
+
 template <int B, bool FP, int M>
 MatrixFn pick_m(bool ow) {
     constexpr int D = dwords_for(B, M);
@@ -354,15 +363,21 @@ hipError_t launch_segment(const MatrixJob &job, MatrixArgs a, size_t k0, size_t 
             a.m = (uint32_t)mn;
         }
         a.ngroups = (uint32_t)((mn + mt - 1) / mt);
-        const int d = nb == 1 && mt <= SMALL_MAX_ROWS ? small_dwords(ncols, a.ngroups) : dwords_for(nb, mt);
+        int d = nb == 1 && mt <= SMALL_MAX_ROWS ? small_dwords(ncols, a.ngroups) : dwords_for(nb, mt);
+        // The wide-tile row launch only where 1 KiB tiles would already
+        // give SMALL_WAVES waves.
+        const bool wide = nb == 1 && mt == 1 && ow && kn <= WIDE_ROW_MAX_K && d == 4;
+        if (wide) d = WIDE_ROW_DWORDS;
         const uint64_t block_bytes = (uint64_t)BLOCK / 64 * 256u * d;
         a.ncoltiles = (uint32_t)((ncols + block_bytes - 1) / block_bytes);
         const uint64_t blocks = a.ngroups == 1
                                     ? a.ncoltiles
                                     : (uint64_t)((a.ncoltiles + 7) / 8) * 8u * a.ngroups;
         const size_t lds = (nb == 8 ? std::min<size_t>(kn, stage8(mt)) : kn) * (size_t)mt * lds_record_bytes(nb, mt);
-        hipLaunchKernelGGL(d < 4 ? pick_small(mt, ow, d) : pick_vector(nb, job.fan_paar, mt, ow),
-                           dim3((uint32_t)blocks), dim3(BLOCK), lds, job.stream, a);
+        const MatrixFn fn = wide    ? (MatrixFn)k_matrix<1, false, 1, WIDE_ROW_DWORDS, true>
+                            : d < 4 ? pick_small(mt, ow, d)
+                                    : pick_vector(nb, job.fan_paar, mt, ow);
+        hipLaunchKernelGGL(fn, dim3((uint32_t)blocks), dim3(BLOCK), lds, job.stream, a);
     } else {
         a.ngroups = 1;
         const uint64_t groups = (ncols / nb + 3) / 4;

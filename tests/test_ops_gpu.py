@@ -260,6 +260,37 @@ def test_matrix_lane_widths_of_small_launches(f):
             assert np.array_equal(host(acc), want), (row_len, nrows, "accumulate")
 
 
+@pytest.mark.parametrize("f", [Gf8D, Gf8B, FanPaar8], ids=lambda f: NAMES[f.id])
+def test_wide_tiles_of_one_overwritten_row(f):
+    # One overwritten byte row from one or two sources takes 2 KiB wave tiles
+    # once 1 KiB tiles would give 2048 waves: mul_into, mul_assign and one-
+    # and two-term dot products over rows of 2 MiB and more, with ragged last
+    # tiles (+16, +1040 bytes), a 3-byte shared misalignment (head, body,
+    # tail) and a length just under the switch (2 MiB - 16: 1 KiB tiles).
+    name = NAMES[f.id]
+    c0, c1 = rand_coeffs(f, 2, 0x1100)
+    for row_len in ((2 << 20) - 16, (2 << 20) + 16, (3 << 20) + 1040):
+        for off in (0, 3):
+            s0, s1 = noise(row_len, 0x1200 + off), noise(row_len, 0x1300 + off)
+            buf = lambda a: dev(np.concatenate([np.zeros(off, np.uint8), a]))[off:]
+            d0, d1 = buf(s0), buf(s1)
+            want = np.zeros(row_len, np.uint8)
+            ol.mul_add(name, want, c0, s0)
+            out = buf(noise(row_len, 0x1400))
+            ops.mul_into(f, out, c0, d0)
+            assert np.array_equal(host(out), want), (row_len, off, "mul_into")
+            out = buf(s0)
+            ops.mul_assign(f, out, c0)
+            assert np.array_equal(host(out), want), (row_len, off, "mul_assign")
+            ol.mul_add(name, want, c1, s1)
+            out = buf(noise(row_len, 0x1500))
+            ops.dot_product(f, out, [c0, c1], [d0, d1])
+            assert np.array_equal(host(out), want), (row_len, off, "dot_product")
+            out = buf(noise(row_len, 0x1600))
+            ops.dot_product_with(f, out, ops.Plan.new(f, [c0, c1]), [d0, d1])
+            assert np.array_equal(host(out), want), (row_len, off, "dot_product_with")
+
+
 @pytest.mark.parametrize("f", [Gf8D, Gf32, FanPaar64], ids=lambda f: NAMES[f.id])
 def test_host_buffer_form_matches_device_form(f):
     # The host form pipelines 8 MiB column chunks through three streams and
