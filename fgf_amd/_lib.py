@@ -96,6 +96,8 @@ PROTOTYPES = {
     "fgf_server_result": (c_int, [c_vp, c_u64, ctypes.POINTER(ctypes.c_double),
                                   ctypes.POINTER(c_int)]),
     "fgf_server_stats_get": (c_int, [c_vp, c_vp]),
+    "fgf_server_cancel": (c_int, [c_vp, c_u64]),
+    "fgf_server_wait": (c_int, [c_vp, c_u64, ctypes.c_double, ctypes.POINTER(c_int)]),
     "fgf_server_replay": (c_int, [c_vp, c_vp, c_vp, c_size, c_vp, ctypes.POINTER(ctypes.c_double)]),
     "fgf_server_destroy": (None, [c_vp]),
     "fgf_batch_bytes": (ctypes.c_uint64, [c_vp]),

@@ -11,6 +11,12 @@
 //      launch the persistent kernel on the slot's stream.
 // Launches on different slots overlap on the device, so requests that arrive
 // while others run start as soon as a slot and budget are free.
+//
+// The device work (launch a slot's batch, poll its end) goes through an
+// executor: the HIP one below, or one the caller supplies, over which the
+// same scheduler runs with no device at all. Requests still queued can be
+// cancelled or time out; a batch whose launch or completion fails fails
+// exactly its own requests, and the loop goes on.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -86,7 +92,10 @@ struct PlanKeyHash {
 };
 
 struct PlanEntry {
-    fgf_plan *plan = nullptr;
+    fgf_plan *plan = nullptr;          // device tables (HIP executor only)
+    std::vector<uint64_t> vals;        // k x rows coefficients, (t, j) at t * rows + j
+    uint32_t rows = 0;
+    uint64_t bytes = 0;                // device table bytes the entry stands for
     uint32_t inflight = 0;
     std::list<PlanKey>::iterator lru;
 };
@@ -104,6 +113,13 @@ struct Slot {
     std::vector<Req> reqs;
     std::vector<PlanEntry *> plans;
     uint64_t bytes = 0;
+    // A caller-supplied executor's batch points into these until the slot
+    // is polled finished.
+    std::vector<fgf_request> creqs;
+    std::vector<uint64_t> tickets;
+    std::vector<uint32_t> rows, idx;   // idx: each request's sources, then its targets
+    std::vector<const uint32_t *> srcp, tgtp;
+    std::vector<const uint64_t *> valp;
 };
 
 }  // namespace
@@ -127,6 +143,9 @@ struct fgf_server {
     fgf_server_stats stats{};
     std::unordered_map<PlanKey, PlanEntry, PlanKeyHash> cache;
     std::list<PlanKey> lru;
+    uint64_t plan_bytes = 0;
+    bool ext = false;                  // a caller-supplied executor
+    fgf_server_executor exec{};
     std::thread worker;
 };
 
@@ -224,43 +243,77 @@ bool plan_values(const Req &q, std::vector<uint64_t> &vals, uint32_t &rows) {
     return true;
 }
 
-PlanEntry *acquire_plan(fgf_server *s, const Req &q) {
+// Device bytes of a plan's tables: entry_dwords(B) per coefficient.
+uint64_t plan_table_bytes(int f, uint32_t k, uint32_t rows) {
+    return (uint64_t)k * rows * entry_dwords(field_bytes(f)) * sizeof(uint32_t);
+}
+
+void evict(fgf_server *s, std::list<PlanKey>::reverse_iterator victim) {
+    auto ce = s->cache.find(*victim);
+    fgf_plan_free(ce->second.plan);
+    s->plan_bytes -= ce->second.bytes;
+    s->lru.erase(std::next(victim).base());
+    s->cache.erase(ce);
+    ++s->stats.plan_evictions;
+    s->stats.plan_cache_bytes = s->plan_bytes;
+}
+
+// The request's plan from the cache, or built and cached (evicting
+// least-recently-used plans that no launch is reading). FGF_OK with *out
+// set; 1 when the plan fits the byte cap only after plans in flight are
+// released (the request waits); negative when the request fails: FGF_EINVAL
+// for a singular decode matrix, FGF_ENOMEM when the plan exceeds the cap on
+// its own or its device tables cannot be allocated.
+int acquire_plan(fgf_server *s, const Req &q, PlanEntry **out) {
     const PlanKey key = plan_key(q);
     auto it = s->cache.find(key);
     if (it != s->cache.end()) {
         s->lru.splice(s->lru.begin(), s->lru, it->second.lru);
         ++s->stats.plan_hits;
         ++it->second.inflight;
-        return &it->second;
+        *out = &it->second;
+        return FGF_OK;
     }
-    ++s->stats.plan_misses;
     std::vector<uint64_t> vals;
     uint32_t rows = 0;
-    if (!plan_values(q, vals, rows)) return nullptr;
-    const int nb = field_bytes(q.r.field);
-    std::vector<uint8_t> raw(vals.size() * nb);
-    for (size_t i = 0; i < vals.size(); ++i)
-        for (int b = 0; b < nb; ++b) raw[i * nb + b] = (uint8_t)(vals[i] >> (8 * b));
-    fgf_plan *p = nullptr;
-    if (fgf_plan_matrix((fgf_field)q.r.field, q.r.k, rows, raw.data(), vals.size(), &p) != FGF_OK)
-        return nullptr;
-    // Evict least-recently-used plans that no launch is reading.
-    while (s->cache.size() >= s->cfg.plan_cache_entries) {
+    if (!plan_values(q, vals, rows)) return FGF_EINVAL;
+    const uint64_t bytes = plan_table_bytes(q.r.field, q.r.k, rows);
+    const uint64_t cap = s->cfg.plan_cache_bytes;
+    if (cap && bytes > cap) return FGF_ENOMEM;
+    auto over = [&] {
+        return s->cache.size() >= s->cfg.plan_cache_entries || (cap && s->plan_bytes + bytes > cap);
+    };
+    while (over()) {
         auto victim = std::find_if(s->lru.rbegin(), s->lru.rend(),
                                    [&](const PlanKey &k2) { return s->cache[k2].inflight == 0; });
         if (victim == s->lru.rend()) break;
-        auto ce = s->cache.find(*victim);
-        fgf_plan_free(ce->second.plan);
-        s->lru.erase(std::next(victim).base());
-        s->cache.erase(ce);
-        ++s->stats.plan_evictions;
+        evict(s, victim);
     }
+    // Every remaining plan is in flight: wait for their launches (the entry
+    // count may run over instead, as it always could).
+    if (cap && s->plan_bytes + bytes > cap) return 1;
+    fgf_plan *p = nullptr;
+    if (!s->ext) {
+        const int nb = field_bytes(q.r.field);
+        std::vector<uint8_t> raw(vals.size() * nb);
+        for (size_t i = 0; i < vals.size(); ++i)
+            for (int b = 0; b < nb; ++b) raw[i * nb + b] = (uint8_t)(vals[i] >> (8 * b));
+        const int rc = fgf_plan_matrix((fgf_field)q.r.field, q.r.k, rows, raw.data(), vals.size(), &p);
+        if (rc != FGF_OK) return rc == FGF_EINVAL ? FGF_EINVAL : FGF_ENOMEM;
+    }
+    ++s->stats.plan_misses;
     s->lru.push_front(key);
     PlanEntry &e = s->cache[key];
     e.plan = p;
+    e.vals = std::move(vals);
+    e.rows = rows;
+    e.bytes = bytes;
     e.inflight = 1;
     e.lru = s->lru.begin();
-    return &e;
+    s->plan_bytes += bytes;
+    s->stats.plan_cache_bytes = s->plan_bytes;
+    *out = &e;
+    return FGF_OK;
 }
 
 template <typename T>
@@ -295,6 +348,7 @@ void finish_slot(fgf_server *s, Slot &sl, int status) {
     sl.plans.clear();
     sl.bytes = 0;
     sl.busy = false;
+    s->cv_done.notify_all();
 }
 
 constexpr size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
@@ -389,22 +443,101 @@ int launch_slot(fgf_server *s, Slot &sl) {
     return FGF_OK;
 }
 
+// Ends a request that never reached a launch (lock held; the caller takes
+// it off its queue).
+void end_unlaunched(fgf_server *s, const Req &q, int status) {
+    s->latency[q.ticket] = std::chrono::duration<double>(Clock::now() - q.submitted).count();
+    s->status[q.ticket] = status;
+    if (status == FGF_ECANCELED)
+        ++s->stats.cancelled;
+    else if (status == FGF_ETIMEDOUT)
+        ++s->stats.timed_out;
+    else
+        ++s->stats.failed;
+    --s->queued;
+    s->cv_done.notify_all();
+}
+
+// Hands a slot's admitted requests to the caller-supplied executor.
+int launch_slot_ext(fgf_server *s, Slot &sl) {
+    const size_t n = sl.reqs.size();
+    sl.creqs.resize(n);
+    sl.tickets.resize(n);
+    sl.rows.resize(n);
+    sl.srcp.resize(n);
+    sl.tgtp.resize(n);
+    sl.valp.resize(n);
+    sl.idx.clear();
+    for (size_t i = 0; i < n; ++i) {
+        const Req &q = sl.reqs[i];
+        if (!q.r.reconstruct) {
+            for (uint32_t t = 0; t < q.r.k + q.r.m; ++t) sl.idx.push_back(t);
+        } else {
+            for (uint32_t t : survivors_of(q)) sl.idx.push_back(t);
+            for (uint32_t e : q.erased) sl.idx.push_back(e);
+        }
+    }
+    size_t off = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const Req &q = sl.reqs[i];
+        const PlanEntry *pe = sl.plans[i];
+        sl.creqs[i] = q.r;
+        sl.creqs[i].shards = q.shards.data();
+        sl.creqs[i].erased = q.erased.empty() ? nullptr : q.erased.data();
+        sl.tickets[i] = q.ticket;
+        sl.rows[i] = pe->rows;
+        sl.srcp[i] = sl.idx.data() + off;
+        sl.tgtp[i] = sl.idx.data() + off + q.r.k;
+        sl.valp[i] = pe->vals.data();
+        off += q.r.k + pe->rows;
+    }
+    const fgf_server_batch b{(uint32_t)(&sl - s->slots.data()), (fgf_field)sl.reqs.front().r.field, n,
+                             sl.creqs.data(), sl.tickets.data(), sl.rows.data(), sl.srcp.data(),
+                             sl.tgtp.data(), sl.valp.data()};
+    const int rc = s->exec.launch(s->exec.ctx, &b);
+    if (rc > 0) return sfail("executor launch returned %d", rc);
+    if (rc == FGF_OK) sl.busy = true;
+    return rc;
+}
+
+// 1 while the slot's batch runs, FGF_OK when it has finished, negative when
+// it failed (lock held).
+int poll_slot(fgf_server *s, Slot &sl) {
+    if (s->ext) {
+        const int r = s->exec.poll(s->exec.ctx, (uint32_t)(&sl - s->slots.data()));
+        return r > 0 ? 1 : r;
+    }
+    const hipError_t q = hipEventQuery(sl.done);
+    return q == hipSuccess ? FGF_OK : q == hipErrorNotReady ? 1 : FGF_EDEVICE;
+}
+
+// Queued requests older than the queue time-out (lock held). Each field's
+// queue is in submission order, so only fronts can have expired first.
+void expire_queued(fgf_server *s) {
+    if (s->cfg.queue_timeout_s <= 0) return;
+    const Clock::time_point now = Clock::now();
+    for (std::deque<Req> &fq : s->queues)
+        while (!fq.empty() &&
+               std::chrono::duration<double>(now - fq.front().submitted).count() > s->cfg.queue_timeout_s) {
+            end_unlaunched(s, fq.front(), FGF_ETIMEDOUT);
+            fq.pop_front();
+        }
+}
+
 void scheduler(fgf_server *s) {
-    (void)hipSetDevice(s->device);
+    if (!s->ext) (void)hipSetDevice(s->device);
     std::unique_lock<std::mutex> lk(s->mu);
     for (;;) {
         bool any_busy = false;
         for (Slot &sl : s->slots) {
             if (!sl.busy) continue;
-            const hipError_t q = hipEventQuery(sl.done);
-            if (q == hipSuccess)
-                finish_slot(s, sl, 0);
-            else if (q != hipErrorNotReady)
-                finish_slot(s, sl, FGF_EDEVICE);
-            else
+            const int r = poll_slot(s, sl);
+            if (r == 1)
                 any_busy = true;
+            else
+                finish_slot(s, sl, r);
         }
-        if (s->stats.completed + s->stats.failed == s->next_ticket) s->cv_done.notify_all();
+        expire_queued(s);
         Slot *free_slot = nullptr;
         for (Slot &sl : s->slots)
             if (!sl.busy) {
@@ -425,23 +558,24 @@ void scheduler(fgf_server *s) {
                 if (s->inflight_bytes + q.bytes > s->cfg.hbm_budget_bytes) {
                     if (free_slot->reqs.empty() && !any_busy) {
                         // Larger than the whole budget: can never be admitted.
-                        s->latency[q.ticket] = 0;
-                        s->status[q.ticket] = FGF_ENOMEM;
-                        ++s->stats.failed;
+                        end_unlaunched(s, q, FGF_ENOMEM);
                         fq.pop_front();
-                        --s->queued;
                         continue;
                     }
                     ++s->stats.admission_waits;
                     break;  // in-order admission: wait for budget
                 }
-                PlanEntry *pe = acquire_plan(s, q);
-                if (!pe) {
-                    s->latency[q.ticket] = 0;
-                    s->status[q.ticket] = FGF_EINVAL;
-                    ++s->stats.failed;
+                PlanEntry *pe = nullptr;
+                const int prc = acquire_plan(s, q, &pe);
+                if (prc == 1) {
+                    // The plan cache's byte cap is held by plans in flight
+                    // (this slot's too, launched below): wait for them.
+                    ++s->stats.admission_waits;
+                    break;
+                }
+                if (prc != FGF_OK) {
+                    end_unlaunched(s, q, prc);
                     fq.pop_front();
-                    --s->queued;
                     continue;
                 }
                 s->inflight_bytes += q.bytes;
@@ -455,10 +589,10 @@ void scheduler(fgf_server *s) {
             if (!free_slot->reqs.empty()) {
                 // Only this thread touches slots: submissions proceed meanwhile.
                 lk.unlock();
-                const int rc = launch_slot(s, *free_slot);
+                const int rc = s->ext ? launch_slot_ext(s, *free_slot) : launch_slot(s, *free_slot);
                 lk.lock();
                 if (rc != FGF_OK) {
-                    finish_slot(s, *free_slot, FGF_EDEVICE);
+                    finish_slot(s, *free_slot, rc < 0 ? rc : FGF_EDEVICE);
                 } else {
                     ++s->stats.launches;
                     s->stats.stripes_launched += free_slot->reqs.size();
@@ -468,13 +602,15 @@ void scheduler(fgf_server *s) {
         }
         if (s->stop && !s->queued && !any_busy && !launched) break;
         if (!launched) {
-            if (any_busy) {
+            if (any_busy && !s->ext) {
                 // A launch is in flight: poll its completion event at once
                 // again (a timed wait would oversleep by tens of us). The
                 // scheduler thread spins while the device works.
                 lk.unlock();
                 std::this_thread::yield();
                 lk.lock();
+            } else if (any_busy) {
+                s->cv_work.wait_for(lk, std::chrono::microseconds(200));
             } else {
                 s->cv_work.wait_for(lk, std::chrono::milliseconds(5));
             }
@@ -498,12 +634,25 @@ int fgf_server_create(const fgf_server_config *cfg, fgf_server **out) {
     if (cfg->subtiles < 1 || cfg->subtiles > 64) return sfail("fgf_server_create: subtiles must be in 1..64");
     if (cfg->waves_per_cu < 1 || cfg->waves_per_cu > 32)
         return sfail("fgf_server_create: waves_per_cu must be in 1..32");
+    if (!(cfg->queue_timeout_s >= 0) || cfg->queue_timeout_s > 1e9)
+        return sfail("fgf_server_create: queue_timeout_s must be a number of seconds >= 0");
+    if (cfg->executor && (!cfg->executor->launch || !cfg->executor->poll))
+        return sfail("fgf_server_create: executor needs both launch and poll");
     auto *s = new (std::nothrow) fgf_server;
     if (!s) return sfail("out of host memory");
     s->cfg = *cfg;
+    s->cfg.executor = nullptr;
+    s->slots.resize(cfg->slots);
+    if (cfg->executor) {
+        // No device resources: the executor does the device work.
+        s->ext = true;
+        s->exec = *cfg->executor;
+        s->worker = std::thread(scheduler, s);
+        *out = s;
+        return FGF_OK;
+    }
     (void)hipGetDevice(&s->device);
     (void)hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, s->device);
-    s->slots.resize(cfg->slots);
     // Descriptor blocks sized for a full batch up front: growing one later
     // frees memory, and hipFree waits for every stream of the device.
     const size_t want = blob_bytes((size_t)cfg->max_batch_stripes * 32, cfg->max_batch_stripes,
@@ -602,8 +751,43 @@ int fgf_server_submit(fgf_server *s, const fgf_request *r, uint64_t *ticket) {
 int fgf_server_drain(fgf_server *s) {
     if (!s) return sfail("fgf_server_drain: null server");
     std::unique_lock<std::mutex> lk(s->mu);
-    s->cv_done.wait(lk, [&] { return s->stats.completed + s->stats.failed == s->next_ticket; });
-    return s->stats.failed ? FGF_EDEVICE : FGF_OK;
+    s->cv_done.wait(lk, [&] {
+        const fgf_server_stats &st = s->stats;
+        return st.completed + st.failed + st.cancelled + st.timed_out == s->next_ticket;
+    });
+    return s->stats.failed ? FGF_EDEVICE : s->stats.timed_out ? FGF_ETIMEDOUT : FGF_OK;
+}
+
+int fgf_server_cancel(fgf_server *s, uint64_t ticket) {
+    if (!s) return sfail("fgf_server_cancel: null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (ticket >= s->next_ticket)
+        return sfail("fgf_server_cancel: unknown ticket %llu", (unsigned long long)ticket);
+    for (std::deque<Req> &fq : s->queues)
+        for (auto it = fq.begin(); it != fq.end(); ++it)
+            if (it->ticket == ticket) {
+                end_unlaunched(s, *it, FGF_ECANCELED);
+                fq.erase(it);
+                return FGF_OK;
+            }
+    return sfail("fgf_server_cancel: ticket %llu is not queued (already launched or finished)",
+                 (unsigned long long)ticket);
+}
+
+int fgf_server_wait(fgf_server *s, uint64_t ticket, double timeout_s, int *status) {
+    if (!s) return sfail("fgf_server_wait: null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (ticket >= s->latency.size())
+        return sfail("fgf_server_wait: unknown ticket %llu", (unsigned long long)ticket);
+    const auto done = [&] { return s->latency[ticket] >= 0; };
+    if (timeout_s < 0 || timeout_s > 1e9) {
+        s->cv_done.wait(lk, done);
+    } else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), done)) {
+        fgf::set_last_error("fgf_server_wait: timed out");
+        return FGF_ETIMEDOUT;
+    }
+    if (status) *status = s->status[ticket];
+    return FGF_OK;
 }
 
 int fgf_server_result(fgf_server *s, uint64_t ticket, double *latency_s, int *status) {

@@ -69,6 +69,8 @@ typedef enum fgf_field {
 #define FGF_EINVAL (-1)  /* misuse: the reference would panic */
 #define FGF_EDEVICE (-2) /* HIP runtime failure */
 #define FGF_ENOMEM (-3)  /* device allocation failure */
+#define FGF_ECANCELED (-4) /* service: the request was cancelled while queued */
+#define FGF_ETIMEDOUT (-5) /* service: the request, or a wait for it, timed out */
 
 /* Thread-local text of the last error (the reference's panic message). */
 const char *fgf_last_error(void);
@@ -363,15 +365,6 @@ void fgf_batch_free(fgf_batch *batch);
  * every shard must be distinct; that is checked at submission.
  * ---------------------------------------------------------------------- */
 typedef struct fgf_server fgf_server;
-typedef struct fgf_server_config {
-    uint64_t hbm_budget_bytes;     /* admitted working-set bytes, (k + m) * row_len each */
-    uint32_t slots;                /* concurrent launches in flight (streams), 1..16 */
-    uint32_t max_batch_stripes;    /* requests per launch, >= 1 */
-    uint32_t plan_cache_entries;   /* device plans kept, >= 1 */
-    int dwords, rows_per_item, subtiles, waves_per_cu; /* kernel shape, as fgf_batch_configure;
-                                    4- and 8-byte fields take at least 8 dwords and their
-                                    own rows per item (4 and 2) */
-} fgf_server_config;
 typedef struct fgf_request {
     fgf_field field;               /* any field; one launch serves one field */
     int reconstruct;               /* 0: write the m parity shards; 1: rebuild `erased` */
@@ -381,23 +374,83 @@ typedef struct fgf_request {
     const uint32_t *erased;        /* reconstruct: shard indices to rebuild (1..m of them) */
     uint32_t nerased;
 } fgf_request;
+/* Executor seam: the scheduler's device work. With none configured the
+ * service launches the batched kernel on each slot's stream and polls the
+ * slot's completion event. A caller-supplied executor replaces both, and
+ * the service then touches no device at all (no streams, buffers or device
+ * plans): admission, the plan cache, cancellation, time-outs and failure
+ * handling run unchanged over it, so they can be driven without a GPU and
+ * with errors injected at chosen calls. One batch holds admitted requests
+ * of one field; request i computes
+ *   shard targets[i][j] = sum_t values[i][t * rows[i] + j] * shard sources[i][t]
+ * (t < k, j < rows[i]) over the shard pointers of reqs[i]. Every array
+ * stays valid until poll reports the slot finished. The callbacks run on
+ * the scheduler thread and must not call fgf_server_* functions. */
+typedef struct fgf_server_batch {
+    uint32_t slot;                       /* 0 .. slots - 1 */
+    fgf_field field;
+    size_t n;                            /* requests */
+    const fgf_request *reqs;
+    const uint64_t *tickets;
+    const uint32_t *rows;                /* rows written: m (encode) or nerased (reconstruct) */
+    const uint32_t *const *sources;      /* k shard indices read */
+    const uint32_t *const *targets;      /* rows[i] shard indices written */
+    const uint64_t *const *values;       /* k x rows[i] coefficients, (t, j) at t * rows[i] + j */
+} fgf_server_batch;
+typedef struct fgf_server_executor {
+    void *ctx;
+    /* FGF_OK: the batch runs on `batch->slot`; negative: every request of
+     * the batch fails with that status. */
+    int (*launch)(void *ctx, const fgf_server_batch *batch);
+    /* 1: still running; FGF_OK: finished; negative: every request of the
+     * slot's batch fails with that status (a lost device, say). */
+    int (*poll)(void *ctx, uint32_t slot);
+} fgf_server_executor;
+typedef struct fgf_server_config {
+    uint64_t hbm_budget_bytes;     /* admitted working-set bytes, (k + m) * row_len each */
+    uint32_t slots;                /* concurrent launches in flight (streams), 1..16 */
+    uint32_t max_batch_stripes;    /* requests per launch, >= 1 */
+    uint32_t plan_cache_entries;   /* device plans kept, >= 1 */
+    int dwords, rows_per_item, subtiles, waves_per_cu; /* kernel shape, as fgf_batch_configure;
+                                    4- and 8-byte fields take at least 8 dwords and their
+                                    own rows per item (4 and 2) */
+    double queue_timeout_s;        /* > 0: a request still queued this long after submission
+                                      fails with FGF_ETIMEDOUT; 0: no limit */
+    uint64_t plan_cache_bytes;     /* > 0: device bytes the cached plans' tables may hold
+                                      (k * rows * 20 * 3^log2(B) each). A plan that does not
+                                      fit beside the plans in flight waits for them; one that
+                                      cannot fit at all fails its request with FGF_ENOMEM,
+                                      as a failed device allocation does. 0: no limit */
+    const fgf_server_executor *executor; /* NULL: the HIP executor; else copied at creation */
+} fgf_server_config;
 typedef struct fgf_server_stats {
     uint64_t submitted, completed, failed, bytes, launches, stripes_launched;
     uint64_t admission_waits, plan_hits, plan_misses, plan_evictions;
     uint64_t max_inflight_bytes;
+    uint64_t cancelled, timed_out, plan_cache_bytes;
 } fgf_server_stats;
 int fgf_server_create(const fgf_server_config *config, fgf_server **out);
 /* Thread-safe; validates and queues. `ticket` numbers requests 0, 1, 2, ... */
 int fgf_server_submit(fgf_server *server, const fgf_request *request, uint64_t *ticket);
-/* Blocks until every submitted request has completed. */
+/* Blocks until every submitted request has finished (completed, failed,
+ * cancelled or timed out). FGF_EDEVICE if any failed, else FGF_ETIMEDOUT if
+ * any timed out, else FGF_OK. */
 int fgf_server_drain(fgf_server *server);
-/* Seconds from submission to observed completion of a completed ticket
+/* Seconds from submission to the observed end of a finished ticket
  * (negative while pending); status 0 ok, negative on failure. */
 int fgf_server_result(fgf_server *server, uint64_t ticket, double *latency_s, int *status);
+/* Removes a still-queued request: its status becomes FGF_ECANCELED. A
+ * request already admitted to a launch, or finished, cannot be cancelled
+ * (FGF_EINVAL). */
+int fgf_server_cancel(fgf_server *server, uint64_t ticket);
+/* Waits up to timeout_s seconds (negative: without limit) for a ticket to
+ * finish: FGF_OK with its status in *status, or FGF_ETIMEDOUT. */
+int fgf_server_wait(fgf_server *server, uint64_t ticket, double timeout_s, int *status);
 int fgf_server_stats_get(fgf_server *server, fgf_server_stats *out);
 /* Open-loop driver: submits requests[i] at arrival_s[i] seconds after the
  * call starts (a client thread spinning on the clock), waits for all, and
- * writes per-request latency from the scheduled arrival to completion. */
+ * writes per-request latency from the scheduled arrival to completion.
+ * fgf_server_destroy waits for launches in flight and queued requests. */
 int fgf_server_replay(fgf_server *server, const fgf_request *requests, const double *arrival_s,
                       size_t n, double *latency_s, double *wall_s);
 void fgf_server_destroy(fgf_server *server);

@@ -109,3 +109,36 @@ def test_overlapping_shards_are_refused_at_submit():
     with pytest.raises(Panic, match="shards overlap"):
         srv.submit(Request(Gf8D, 4, 2, shards, L))
     srv.close()
+
+
+def test_plan_cache_byte_cap_on_device_plans():
+    # A cap with room for the largest plan but not for two: distinct
+    # matrices are served one plan at a time (each evicts the last), every
+    # result exact; a plan over the cap fails its request with ENOMEM while
+    # the others are still served.
+    from fgf_amd import serve
+    specs = workloads.draw_serving_mix([(Gf8D, 1.0)], 24, seed=21, n_matrices=6, lmin=4096,
+                                       lmax=32 << 10)
+    pool, expected = workloads.materialize_serving(specs, seed=22)
+    plan_bytes = lambda sp: sp.k * (len(sp.erased) or sp.m) * 20
+    cap = max(plan_bytes(sp) for sp in specs) + 1
+    srv = Server(slots=2, max_batch_stripes=4, plan_cache_bytes=cap)
+    srv.replay([workloads.serve_request(pool, sp) for sp in specs], [0.0] * len(specs))
+    for sp, want in zip(specs, expected):
+        assert torch.equal(outputs(pool, sp), want)
+    st = srv.stats()
+    assert st["failed"] == 0 and st["plan_cache_bytes"] <= cap and st["plan_evictions"] > 0
+    srv.close()
+    small = min(specs, key=plan_bytes)
+    clear(pool, specs)
+    srv = Server(slots=1, plan_cache_bytes=plan_bytes(small))
+    tickets = [srv.submit(workloads.serve_request(pool, sp)) for sp in specs]
+    with pytest.raises(DeviceError):
+        srv.drain()
+    for sp, t, want in zip(specs, tickets, expected):
+        status = srv.result(t)[1]
+        if plan_bytes(sp) > plan_bytes(small):
+            assert status == serve.ENOMEM
+        else:
+            assert status == serve.OK and torch.equal(outputs(pool, sp), want)
+    srv.close()
