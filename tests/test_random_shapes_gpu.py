@@ -205,3 +205,71 @@ def test_random_vector_op_matches_oracle(case):
     assert (got[:o] == 0xA5).all() and (got[o + len(want):] == 0xA5).all(), "guard bytes changed"
     bad = np.flatnonzero(got[o:o + len(want)] != want)
     assert bad.size == 0, f"{bad.size} bytes differ, first at {bad[0]}"
+
+
+def draw_batch(seed: int) -> dict:
+    rng = np.random.default_rng(seed)
+    f = ALL_FIELDS[int(rng.integers(len(ALL_FIELDS)))]
+    b = f.BYTES
+    dwords = int(rng.choice([4, 8])) if b <= 2 else 8
+    rows = int(rng.choice([4, 8])) if b <= 2 else (4 if b == 4 else 2)
+    cfg = (dwords, rows, int(rng.choice([1, 2, 4, 9])), int(rng.choice([2, 8, 12, 20])), int(rng.integers(2)))
+    return dict(f=f, n=int(rng.integers(4, 28)), cfg=cfg, seed=seed)
+
+
+BATCH_CASES = [draw_batch(0xBA7C00 + i) for i in range(18)]
+
+
+@pytest.mark.parametrize("case", BATCH_CASES, ids=lambda c: "{}-n{}-cfg{}-{}".format(
+    NAMES[c["f"].id], c["n"], "_".join(map(str, c["cfg"])), c["seed"] & 0xff))
+def test_random_batch_matches_oracle(case):
+    """One batched launch (and its graph replay) over stripes of random
+    k <= 32, m <= 12 and row length, overwriting and accumulating stripes
+    mixed, some sources shared between stripes, at a random tile
+    configuration: every stripe's rows equal the oracle's, bytes past them
+    untouched."""
+    from fgf_amd.batch import Batch, Stripe
+    f, n, name = case["f"], case["n"], NAMES[case["f"].id]
+    rng = np.random.default_rng(case["seed"] ^ 0x5A5A)
+    b = f.BYTES
+    shared_h = [ol.noise(16384, case["seed"] + 100 + i) for i in range(4)]
+    shared_d = [torch.from_numpy(s).cuda() for s in shared_h]
+    per_stripe = BUDGET[b] // n
+    stripes, checks = [], []
+    for s in range(n):
+        k, m = int(rng.integers(1, 33)), int(rng.integers(1, 13))
+        L = 16 * int(rng.integers(1, max(2, min(16384, per_stripe * b // (k * m)) // 16 + 1)))
+        raw = ol.noise(k * m * b, case["seed"] * 31 + s)
+        vals = [int.from_bytes(raw[i * b:(i + 1) * b].tobytes(), "little") for i in range(k * m)]
+        srcs_h, srcs_d = [], []
+        for t in range(k):
+            if rng.random() < 0.3:
+                i = int(rng.integers(len(shared_h)))
+                srcs_h.append(shared_h[i][:L])
+                srcs_d.append(shared_d[i])
+            else:
+                h = ol.noise(L, case["seed"] * 977 + s * 64 + t)
+                srcs_h.append(h)
+                srcs_d.append(torch.from_numpy(h).cuda())
+        overwrite = bool(rng.random() < 0.6)
+        init = ol.noise(m * L, case["seed"] + 7 * s) if not overwrite else np.full(m * L, 0x3C, np.uint8)
+        whole = torch.full((m * L + GUARD,), 0xA5, dtype=torch.uint8, device="cuda")
+        want = init.copy() if not overwrite else np.zeros(m * L, np.uint8)
+        ol.mul_add_matrix(name, want, L, m, vals, srcs_h, overwrite=overwrite)
+        plan = ops.Plan.matrix(f, k, m, vals)
+        rows = [whole[j * L:(j + 1) * L] for j in range(m)]
+        stripes.append(Stripe(plan, srcs_d, rows, L, overwrite))
+        checks.append((whole, init, want))
+    batch = Batch(f, stripes)
+    batch.configure(*case["cfg"])
+    for runner in (batch.run, batch.run_graph, batch.run_graph):
+        for whole, init, _ in checks:
+            whole[:len(init)] = torch.from_numpy(init).cuda()
+        runner()
+        torch.cuda.synchronize()
+        for s, (whole, init, want) in enumerate(checks):
+            got = whole.cpu().numpy()
+            assert (got[len(want):] == 0xA5).all(), (runner.__name__, s, "guard bytes changed")
+            bad = np.flatnonzero(got[:len(want)] != want)
+            assert bad.size == 0, (runner.__name__, s, stripes[s].plan.dimensions(), stripes[s].row_len,
+                                   stripes[s].overwrite, int(bad[0]))
