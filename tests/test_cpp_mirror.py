@@ -1,7 +1,8 @@
 """The C++ mirror of `fgf::ops` (include/fgf_amd.hpp) and its tests, run as
-binaries: tests/cpp/ops_parity.cpp (parity, named after tests/ops.rs) and
+binaries: tests/cpp/ops_parity.cpp (parity, named after tests/ops.rs),
 tests/cpp/zero_alloc.cpp (steady-state host allocations, after
-tests/zero_alloc.rs).
+tests/zero_alloc.rs) and tests/cpp/serve_threads.cpp (the serving loop under
+concurrent clients through its executor seam, on the CPU).
 
 The CPU test runs the misuse cases, which fail validation before any device
 access; the GPU tests run every parity case against the oracle on a gfx950
@@ -42,6 +43,17 @@ def _all_passed(out: str) -> int:
 def test_cpp_mirror_misuse_panics_with_reference_text():
     out = _run("ops_parity", ["--cpu"], 60)
     assert _all_passed(out) == 12
+
+
+@pytest.mark.parametrize("args", [[], ["3000", "0.002"]], ids=["fast", "time-outs"])
+def test_serving_loop_accounts_for_every_request_under_concurrent_clients(args):
+    out = _run("serve_threads", args, 120)
+    assert "counts consistent" in out
+    stats = dict(re.findall(r"(\w+) (\d+)", out.splitlines()[-2]))
+    assert stats["submitted"] == "600" and int(stats["completed"]) > 0 and int(stats["failed"]) > 0
+    assert int(stats["cancelled"]) > 0 and stats["bad"] == "0"
+    if args:
+        assert int(stats["timed_out"]) > 0
 
 
 @pytest.mark.gpu
