@@ -2,29 +2,33 @@
 // (ops::mul_elementwise, src/ops.rs:953-957; scalar definition
 // src/kernel/scalar.rs:119-130), both operands varying.
 //
-// Vector kernel. Each lane holds 32 bytes of a and of b (two 16-byte pieces
-// a wave-tile of 1 KiB apart, so every load instruction covers 1 KiB
-// contiguous) and transposes them into byte planes of four elements each.
-// A product of two tower elements is Karatsuba all the way down to bytes:
-// with x = x0 + x1*u and y = y0 + y1*u over the half field,
+// Byte fields (Gf8B, Gf8D, fp8): each byte product of two variable bytes is
+// one lookup in a 64 KiB table T[a*256 + b] = a*b held in LDS, its address
+// (b | a << 8) formed by one v_perm. Each lane holds 32 bytes of a and of b
+// (two 16-byte pieces a wave-tile of 1 KiB apart, so every load instruction
+// covers 1 KiB contiguous).
+//
+// 2-, 4- and 8-byte fields: bit-sliced, 32 pairs per lane at a time
+// (bitslice.hpp). A product of two tower elements is Karatsuba down to
+// bytes, with x = x0 + x1*u and y = y0 + y1*u over the half field,
 //   GF(2^2^n) towers (u^2 = u + DELTA, gf16.rs:143-153, gf32.rs:121-129,
 //     gf64.rs:120-128): lo = x0 y0 + DELTA x1 y1, hi = (x0+x1)(y0+y1) + x0 y0
 //   Fan-Paar (X^2 = alpha X + 1, fan_paar.rs:124-145): lo = x0 y0 + x1 y1,
 //     hi = (x0+x1)(y0+y1) + x0 y0 + x1 y1 + alpha x1 y1
-// so a GF(2^64) product is 27 byte products. Each byte product of two
-// variable bytes is one lookup in a 64 KiB table T[a*256 + b] = a*b held in
-// LDS, its address (b | a << 8) formed by one v_perm. The DELTA and alpha
-// multiplies are by constants known at compile time: they run through the
-// v_perm constant-table machinery of the matrix kernels (Karatsuba byte
-// constants, zero constants dropped at compile time).
+// so a GF(2^64) product is 27 byte products; on bit planes each is 81 VALU
+// bit operations over 32 elements, against one random-bank LDS lookup per
+// product for a table (bank conflicts made those lookups, not the VALU, the
+// bound of the table form for these fields).
 //
 // Generic kernel (heads, tails, buffers whose alignments differ): one element
 // per thread through the byte field's log/exp tables.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <utility>
 
+#include "bitslice.hpp"
 #include "field.hpp"
 #include "kernels.hpp"
 #include "matrix_core.hpp"
@@ -34,66 +38,6 @@ namespace {
 
 using namespace core;
 
-// ---------------------------------------------------------------------------
-// Compile-time field arithmetic for the DELTA / alpha constants.
-// ---------------------------------------------------------------------------
-
-// The Karatsuba byte constants of a constant multiply (kara_constants_of).
-struct KaraConsts {
-    uint8_t c[27];
-};
-
-template <unsigned BITS, bool FP, uint64_t C>
-constexpr KaraConsts kara_consts_of() {
-    KaraConsts k{};
-    kara_constants_of<BITS>(FP, C, k.c);
-    return k;
-}
-
-// The v_perm tables of one byte constant (layout of TAB_DW in kernels.hpp).
-struct ConstTab {
-    uint32_t t0lo, t0hi, t1lo, t1hi, t2;
-};
-
-constexpr ConstTab const_tab(bool fp, uint8_t l) {
-    uint32_t e[TAB_DW] = {};
-    byte_const_table(fp ? FP8 : GF8B, l, e);
-    return {e[0], e[1], e[2], e[3], e[4]};
-}
-
-// ---------------------------------------------------------------------------
-// Device arithmetic on byte planes.
-// ---------------------------------------------------------------------------
-
-template <bool FP, uint8_t L>
-__device__ __forceinline__ uint32_t lookup_const(const Sel &s) {
-    if constexpr (L == 0) {
-        return 0u;
-    } else {
-        constexpr ConstTab t = const_tab(FP, L);
-        return xor3(vperm(t.t0hi, t.t0lo, s.s0), vperm(t.t1hi, t.t1lo, s.s1), vperm(t.t2, t.t2, s.s2));
-    }
-}
-
-template <int H, bool FP, uint64_t C, int... I>
-__device__ __forceinline__ void const_lookups(const Sel *ex, uint32_t *lk,
-                                              std::integer_sequence<int, I...>) {
-    constexpr KaraConsts K = kara_consts_of<8u * H, FP, C>();
-    ((lk[I] = lookup_const<FP, K.c[I]>(ex[I])), ...);
-}
-
-// out (H planes) = C * in (H planes), C an element of the H-byte field.
-template <int H, bool FP, uint64_t C>
-__device__ __forceinline__ void mul_const(const uint32_t *in, uint32_t *out) {
-    Sel base[H], ex[kterms(H)];
-#pragma unroll
-    for (int q = 0; q < H; ++q) base[q] = selectors(in[q]);
-    kara_expand<H, FP>(base, ex);
-    uint32_t lk[kterms(H)];
-    const_lookups<H, FP, C>(ex, lk, std::make_integer_sequence<int, kterms(H)>{});
-    kara_combine<H>(lk, out);
-}
-
 // Four byte products a_i * b_i (i = byte lane) through the LDS table.
 __device__ __forceinline__ uint32_t byte_products(uint32_t a, uint32_t b, const uint8_t *T) {
     const uint32_t r0 = T[vperm(a, b, 0x0c0c0400u)];
@@ -101,41 +45,6 @@ __device__ __forceinline__ uint32_t byte_products(uint32_t a, uint32_t b, const 
     const uint32_t r2 = T[vperm(a, b, 0x0c0c0602u)];
     const uint32_t r3 = T[vperm(a, b, 0x0c0c0703u)];
     return r0 | (r1 << 8) | (r2 << 16) | (r3 << 24);
-}
-
-// out = x * y on B byte planes (four elements per plane).
-template <int B, bool FP>
-__device__ __forceinline__ void mul_planes(const uint32_t *x, const uint32_t *y, uint32_t *out,
-                                           const uint8_t *T) {
-    if constexpr (B == 1) {
-        out[0] = byte_products(x[0], y[0], T);
-    } else {
-        constexpr int H = B / 2;
-        uint32_t p00[H], p11[H], pm[H], sx[H], sy[H], d[H];
-        mul_planes<H, FP>(x, y, p00, T);
-        mul_planes<H, FP>(x + H, y + H, p11, T);
-#pragma unroll
-        for (int q = 0; q < H; ++q) {
-            sx[q] = x[q] ^ x[H + q];
-            sy[q] = y[q] ^ y[H + q];
-        }
-        mul_planes<H, FP>(sx, sy, pm, T);
-        if constexpr (!FP) {
-            mul_const<H, false, tower_delta_of(8u * B)>(p11, d);
-#pragma unroll
-            for (int q = 0; q < H; ++q) {
-                out[q] = p00[q] ^ d[q];
-                out[H + q] = pm[q] ^ p00[q];
-            }
-        } else {
-            mul_const<H, true, fp_alpha(1, 8u * H)>(p11, d);
-#pragma unroll
-            for (int q = 0; q < H; ++q) {
-                out[q] = p00[q] ^ p11[q];
-                out[H + q] = xor3(pm[q], p00[q], p11[q]) ^ d[q];
-            }
-        }
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -179,8 +88,8 @@ __device__ __forceinline__ void build_product_table(uint8_t *T, uint8_t *s_ex, u
     __syncthreads();
 }
 
-// Aligned body: every pointer 16-byte aligned, nbytes a multiple of 16.
-// Each wave step covers EW_TILE bytes as two 1 KiB pieces per operand. The
+// Aligned body of the byte fields: every pointer 16-byte aligned, nbytes a
+// multiple of 16. Each wave step covers EW_TILE bytes as two 1 KiB pieces per operand. The
 // next step's operands are loaded while the current step is multiplied, in
 // two ping-pong register sets. Operands and products go through raw buffer
 // resources sized to the step's bytes before nbytes: a ragged last step
@@ -190,22 +99,10 @@ __device__ __forceinline__ void build_product_table(uint8_t *T, uint8_t *s_ex, u
 template <int F>
 __device__ __forceinline__ void ew_step(uint8_t *dst, const uint32_t (&xa)[8], const uint32_t (&xb)[8],
                                         uint64_t tile, uint32_t n, uint32_t lane, const uint8_t *s_prod) {
-    constexpr int NB = field_bytes(F);
-    constexpr bool FP = is_fan_paar(F);
+    static_assert(field_bytes(F) == 1, "the table kernel multiplies byte fields");
     uint32_t r[8];
-    if constexpr (NB == 1) {
 #pragma unroll
-        for (int d = 0; d < 8; ++d) r[d] = byte_products(xa[d], xb[d], s_prod);
-    } else {
-#pragma unroll
-        for (int g = 0; g < 8 / NB; ++g) {
-            uint32_t pa[NB], pb[NB], pr[NB];
-            to_planes<NB>(&xa[g * NB], pa);
-            to_planes<NB>(&xb[g * NB], pb);
-            mul_planes<NB, FP>(pa, pb, pr, s_prod);
-            from_planes<NB>(pr, &r[g * NB]);
-        }
-    }
+    for (int d = 0; d < 8; ++d) r[d] = byte_products(xa[d], xb[d], s_prod);
     const BufRes rd = buf_res(dst + tile, n);
     bstore16(rd, lane * 16u, &r[0]);
     bstore16(rd, lane * 16u + 1024u, &r[4]);
@@ -243,6 +140,140 @@ __global__ __launch_bounds__(EW_BLOCK) void k_elementwise(uint8_t *dst, const ui
         if (tile + stride < nbytes)
             ew_step<F>(dst, na, nb, tile + stride, bytes_at(tile + stride), lane, s_prod);
         tile += 2 * stride;
+    } while (tile < nbytes);
+}
+
+// ---------------------------------------------------------------------------
+// Bit-sliced kernel for the 2-, 4- and 8-byte fields.
+// ---------------------------------------------------------------------------
+//
+// A lane takes 32 elements per step: 2B pieces of 16 bytes, piece k at
+// k * 1024 + lane * 16 of the wave's tile (every load instruction covers
+// 1 KiB contiguous). Its 8B dwords are turned into the 8B bit planes of the
+// 32 elements in registers, multiplied plane-wise (bitslice.hpp) and turned
+// back. The turn is a register transpose: with the register index R and the
+// bit position P of a dword written in binary, exchanging bit j of R with bit
+// j of P for each j (a delta swap between the register pairs that differ in
+// bit j of R) moves the element index into P and the plane index into R.
+// Each exchange is its own inverse and they commute, so the same exchanges
+// turn the product planes back. Exchanges of 8- and 16-bit blocks are one
+// v_perm per register, those of 1-, 2- and 4-bit blocks a shift and a
+// v_bfi_b32.
+//
+// Register layout before the exchanges (element e, byte q of the element):
+//   B = 8: r[32 h + 2 k + i] = dword h (bytes 4h..4h+3) of element 2k + i
+//          of piece k, so R = (h, k3..k0, i) and P = (q1 q0, bit);
+//          exchanging R bits 0..4 with P bits 0..4 leaves plane 32 h + 8 q +
+//          bit in register 32 h + 8 q + bit.
+//   B = 4: r[4 k + d] = element 4k + d; R bits 0..4 <-> P bits 0..4.
+//   B = 2: r[4 k + d] = elements 8k + 2d (low half) and 8k + 2d + 1;
+//          R bits 0..3 <-> P bits 0..3; P bit 4 stays the element's parity.
+
+constexpr int BS_BLOCK = 256;
+
+template <int NREG, int K, int J>
+__device__ __forceinline__ void exchange(uint32_t *r) {
+#pragma unroll
+    for (int lo = 0; lo < NREG; ++lo) {
+        if (lo & (1 << K)) continue;
+        const int hi = lo | (1 << K);
+        const uint32_t a = r[lo], b = r[hi];
+        if constexpr (J == 4) {
+            r[lo] = vperm(b, a, 0x05040100u);
+            r[hi] = vperm(b, a, 0x07060302u);
+        } else if constexpr (J == 3) {
+            r[lo] = vperm(b, a, 0x06020400u);
+            r[hi] = vperm(b, a, 0x07030501u);
+        } else {
+            constexpr uint32_t M = J == 0 ? 0x55555555u : J == 1 ? 0x33333333u : 0x0F0F0F0Fu;
+            constexpr int S = 1 << J;
+            r[lo] = (a & M) | ((b << S) & ~M);
+            r[hi] = ((a >> S) & M) | (b & ~M);
+        }
+    }
+}
+
+template <int NB>
+__device__ __forceinline__ void transpose_planes(uint32_t *r) {
+    constexpr int NREG = 8 * NB;
+    exchange<NREG, 0, 0>(r);
+    exchange<NREG, 1, 1>(r);
+    exchange<NREG, 2, 2>(r);
+    exchange<NREG, 3, 3>(r);
+    if constexpr (NB >= 4) exchange<NREG, 4, 4>(r);
+}
+
+// Register of dword w (0..3) of piece k.
+template <int NB>
+__device__ __forceinline__ constexpr int piece_reg(int k, int w) {
+    return NB == 8 ? 32 * (w & 1) + 2 * k + (w >> 1) : 4 * k + w;
+}
+
+template <int NB, int POL>
+__device__ __forceinline__ void load_planes(uint32_t *r, BufRes res, uint32_t lane) {
+#pragma unroll
+    for (int k = 0; k < 2 * NB; ++k) {
+        uint32_t w[4];
+        bload16<POL>(w, res, (uint32_t)k * 1024u + lane * 16u);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) r[piece_reg<NB>(k, i)] = w[i];
+    }
+}
+
+template <int NB>
+__device__ __forceinline__ void store_planes(const uint32_t *r, BufRes res, uint32_t lane) {
+#pragma unroll
+    for (int k = 0; k < 2 * NB; ++k) {
+        uint32_t w[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) w[i] = r[piece_reg<NB>(k, i)];
+        bstore16(res, (uint32_t)k * 1024u + lane * 16u, w);
+    }
+}
+
+// PF: the next step's operands are loaded while the current step is
+// multiplied (the 8-byte fields: one wave per SIMD, whose 512 registers hold
+// both steps, and no other wave on the SIMD to cover the load latency).
+template <int F, bool PF>
+__global__ __launch_bounds__(BS_BLOCK) void k_elementwise_bs(uint8_t *dst, const uint8_t *a, const uint8_t *b,
+                                                              uint64_t nbytes) {
+    constexpr int NB = field_bytes(F), N = 8 * NB;
+    constexpr uint32_t TILE = 2048u * NB;  // 64 lanes x 32 elements
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint64_t stride = (uint64_t)gridDim.x * (BS_BLOCK / 64) * TILE;
+    uint64_t tile = ((uint64_t)blockIdx.x * (BS_BLOCK / 64) + wave) * TILE;
+    if (tile >= nbytes) return;
+    auto bytes_at = [&](uint64_t t) -> uint32_t {
+        return t < nbytes ? (uint32_t)(nbytes - t < TILE ? nbytes - t : TILE) : 0u;
+    };
+    // A step past the end loads zeros from an empty resource.
+    auto load = [&](uint32_t *x, uint32_t *y, uint64_t t) {
+        const uint32_t n = bytes_at(t);
+        const uint64_t o = n ? t : 0u;
+        load_planes<NB, BUF_NT>(x, buf_res(a + o, n), lane);
+        load_planes<NB, BUF_NT>(y, buf_res(b + o, n), lane);
+    };
+    uint32_t nx[N], ny[N];
+    if constexpr (PF) load(nx, ny, tile);
+    do {
+        uint32_t x[N], y[N], o[N];
+        if constexpr (PF) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                x[i] = nx[i];
+                y[i] = ny[i];
+            }
+            load(nx, ny, tile + stride);
+        } else {
+            load(x, y, tile);
+        }
+        transpose_planes<NB>(x);
+        transpose_planes<NB>(y);
+        bs::field_mul<F>(x, y, o);
+        transpose_planes<NB>(o);
+        store_planes<NB>(o, buf_res(dst + tile, bytes_at(tile)), lane);
+        tile += stride;
     } while (tile < nbytes);
 }
 
@@ -318,9 +349,45 @@ __global__ __launch_bounds__(GEN_BLOCK) void k_elementwise_generic(uint8_t *dst,
 
 using EwFn = void (*)(uint8_t *, const uint8_t *, const uint8_t *, uint64_t);
 
+// The aligned-body kernel of a field: block size, bytes per wave tile,
+// resident blocks per CU.
+struct EwBody {
+    EwFn fn;
+    unsigned block;
+    uint64_t tile;
+    unsigned per_cu;
+};
+
+// Blocks of a bit-sliced kernel that one CU holds at once (its register
+// use decides), asked of the runtime once per field.
+unsigned bs_blocks_per_cu(EwFn fn, int field) {
+    static std::atomic<unsigned> cache[FIELD_COUNT];
+    unsigned n = cache[field].load(std::memory_order_relaxed);
+    if (n == 0) {
+        int blocks = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void *)fn, BS_BLOCK, 0) != hipSuccess ||
+            blocks <= 0)
+            blocks = 1;
+        n = (unsigned)blocks;
+        cache[field].store(n, std::memory_order_relaxed);
+    }
+    return n;
+}
+
+template <int F>
+EwBody ew_body() {
+    if constexpr (field_bytes(F) >= 2) {
+        constexpr bool PF = field_bytes(F) == 8;
+        const EwFn fn = (EwFn)k_elementwise_bs<F, PF>;
+        return {fn, BS_BLOCK, 2048u * field_bytes(F), bs_blocks_per_cu(fn, F)};
+    } else {
+        return {(EwFn)k_elementwise<F>, EW_BLOCK, EW_TILE, EW_BLOCKS_PER_CU};
+    }
+}
+
 template <int... Fs>
-void pick_ew(int field, EwFn &vec, EwFn &gen, std::integer_sequence<int, Fs...>) {
-    ((field == Fs ? (vec = (EwFn)k_elementwise<Fs>, gen = (EwFn)k_elementwise_generic<Fs>, 0) : 0), ...);
+void pick_ew(int field, EwBody &vec, EwFn &gen, std::integer_sequence<int, Fs...>) {
+    ((field == Fs ? (vec = ew_body<Fs>(), gen = (EwFn)k_elementwise_generic<Fs>, 0) : 0), ...);
 }
 
 int device_cus() {
@@ -358,17 +425,18 @@ hipError_t launch_elementwise(int field, uint8_t *dst, const uint8_t *a, const u
         }
     }
     const uint64_t tail0 = head + body, tail = len - tail0;
-    EwFn vec = nullptr, gen = nullptr;
+    EwBody vec{};
+    EwFn gen = nullptr;
     pick_ew(field, vec, gen, std::make_integer_sequence<int, FIELD_COUNT>{});
     if (head) {
         hipLaunchKernelGGL(gen, dim3(generic_grid(head / nb)), dim3(GEN_BLOCK), 0, stream, dst, a, b,
                            head / nb);
     }
     if (body) {
-        const uint64_t tiles = (body + EW_TILE - 1) / EW_TILE;
-        const uint64_t blocks = std::min<uint64_t>((tiles + EW_BLOCK / 64 - 1) / (EW_BLOCK / 64),
-                                                   (uint64_t)device_cus() * EW_BLOCKS_PER_CU);
-        hipLaunchKernelGGL(vec, dim3((unsigned)blocks), dim3(EW_BLOCK), 0, stream, dst + head,
+        const uint64_t tiles = (body + vec.tile - 1) / vec.tile, waves = vec.block / 64;
+        const uint64_t blocks =
+            std::min<uint64_t>((tiles + waves - 1) / waves, (uint64_t)device_cus() * vec.per_cu);
+        hipLaunchKernelGGL(vec.fn, dim3((unsigned)blocks), dim3(vec.block), 0, stream, dst + head,
                            a + head, b + head, body);
     }
     if (tail) {

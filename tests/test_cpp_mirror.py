@@ -1,8 +1,10 @@
 """The C++ mirror of `fgf::ops` (include/fgf_amd.hpp) and its tests, run as
 binaries: tests/cpp/ops_parity.cpp (parity, named after tests/ops.rs),
 tests/cpp/zero_alloc.cpp (steady-state host allocations, after
-tests/zero_alloc.rs) and tests/cpp/serve_threads.cpp (the serving loop under
-concurrent clients through its executor seam, on the CPU).
+tests/zero_alloc.rs), tests/cpp/serve_threads.cpp (the serving loop under
+concurrent clients through its executor seam, on the CPU) and
+tests/cpp/bitslice_host.cpp (the bit-sliced elementwise arithmetic against
+the oracle, on the CPU).
 
 The CPU test runs the misuse cases, which fail validation before any device
 access; the GPU tests run every parity case against the oracle on a gfx950
@@ -54,6 +56,14 @@ def test_serving_loop_accounts_for_every_request_under_concurrent_clients(args):
     assert int(stats["cancelled"]) > 0 and stats["bad"] == "0"
     if args:
         assert int(stats["timed_out"]) > 0
+
+
+def test_bitsliced_elementwise_arithmetic_matches_oracle():
+    # The fp8 isomorphism on all 65,536 products, the plane byte product on
+    # all pairs, and the six 2-, 4- and 8-byte fields on random, special and
+    # basis operands.
+    out = _run("bitslice_host", [], 60)
+    assert _all_passed(out) == 8
 
 
 @pytest.mark.gpu
