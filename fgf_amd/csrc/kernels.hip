@@ -27,6 +27,23 @@ void build_coeff_table(int field, uint64_t c, uint32_t *out) { coeff_table(field
 
 void build_packed_tables(int field, const uint64_t *vals, size_t k, size_t m, int R, uint8_t *out) {
     const size_t groups = (m + R - 1) / R;
+    if (field_bytes(field) == 4) {
+        // Nine tables per (group, source): Karatsuba position p, record x =
+        // the R byte products kc(t, j)[p] * x.
+        const int bf = byte_field(field);
+        for (size_t g = 0; g < groups; ++g)
+            for (size_t t = 0; t < k; ++t)
+                for (int j = 0; j < R; ++j) {
+                    const size_t row = g * R + j;
+                    uint8_t kc[27] = {};
+                    if (row < m) kara_constants_of<32>(is_fan_paar(field), vals[t * m + row], kc);
+                    for (int p = 0; p < 9; ++p) {
+                        uint8_t *tab = out + ((g * k + t) * 9 + p) * 256u * R;
+                        for (unsigned x = 0; x < 256; ++x) tab[x * R + j] = byte_mul(bf, kc[p], (uint8_t)x);
+                    }
+                }
+        return;
+    }
     if (field_bytes(field) == 2) {
         // Two tables per (group, source): byte position i of the element,
         // record x = the R two-byte products c(t, j) * (x << 8i).
@@ -623,6 +640,167 @@ __global__ __launch_bounds__(PTAB_BLOCK) void k_matrix_ptab2(const PtabArgs a) {
     }
 }
 
+// The packed form for 4-byte elements (GF(2^32), Fan-Paar 32), over the
+// Karatsuba expansion. A coefficient's product is the combine of 9 byte
+// products kc[p] * x[p], where x[p] are the 9 expanded bytes of the source
+// element and kc[p] the 9 Karatsuba constants of the coefficient
+// (kara_constants_of, branch order S, O1, O2 at each level). Per (group of
+// R = 4 or 8 rows, source) nine 256-entry tables, one per position p, hold
+// in record x the R byte products kc(t, j)[p] * x. A lane's 16 column bytes
+// are 4 elements; each takes 9 lookups per source, and acc[e][p] holds its
+// R row bytes of position p. The combine is XOR only (lo = S ^ O1,
+// hi = S ^ O2 at each level), so it runs once per element after the last
+// source:
+//   byte 0 = P0^P1^P3^P4, byte 1 = P0^P2^P3^P5,
+//   byte 2 = P0^P1^P6^P7, byte 3 = P0^P2^P6^P8,
+// then a 4 x 4 byte transpose per row quartet moves the positions into rows.
+// The LDS stage holds kt_stage<R>() sources whose table bases are immediate
+// offsets of the unrolled source loop; a lookup's address is its expanded
+// byte scaled by the record size, and scaling commutes with the XORs of the
+// expansion, so each element costs 4 scaled byte extractions, 5 XORs and
+// 9 lookups.
+// Blocks of 1024 threads (16 KiB of columns) halve the table staging per
+// source byte against 512. The LDS stage is 126 KiB (7 sources) for 8-row
+// records, one block per CU at 98 VGPRs, and 72 KiB (8 sources) for 4-row
+// records, whose 60 VGPRs let two blocks share a CU (measured at k = 32,
+// 16 MiB: m = 8 0.2525 ms against 0.270-0.282 with 72 or 54 KiB stages;
+// m = 4 0.216 ms against 0.226 with 126 KiB).
+constexpr int KT_BLOCK = 1024;
+constexpr int KT_E = 9;
+constexpr size_t ktab_lds(int R) { return (R == 8 ? 126 : 72) * 1024; }
+
+template <int R>
+constexpr int kt_stage() { return (int)(ktab_lds(R) / ((size_t)KT_E * 256 * R)); }
+
+// The 9 expanded bytes of element w, scaled by R (SH = log2 R), in the
+// branch order of kara_constants_of<32>.
+template <bool FP, int SH>
+__device__ __forceinline__ void kt_expand(uint32_t w, uint32_t (&x)[KT_E]) {
+    constexpr uint32_t M = 0xFFu << SH;
+    const uint32_t a0 = (w << SH) & M, a1 = (w >> (8 - SH)) & M;
+    const uint32_t b0 = (w >> (16 - SH)) & M, b1 = (w >> (24 - SH)) & M;
+    const uint32_t a2 = a0 ^ a1, b2 = b0 ^ b1, s0 = a0 ^ b0, s1 = a1 ^ b1, s2 = s0 ^ s1;
+    if constexpr (FP) {
+        // halves (a + b, a, b), each (v0 + v1, v0, v1)
+        x[0] = s2; x[1] = s0; x[2] = s1;
+        x[3] = a2; x[4] = a0; x[5] = a1;
+        x[6] = b2; x[7] = b0; x[8] = b1;
+    } else {
+        // halves (a, b, a + b), each (v0, v1, v0 + v1)
+        x[0] = a0; x[1] = a1; x[2] = a2;
+        x[3] = b0; x[4] = b1; x[5] = b2;
+        x[6] = s0; x[7] = s1; x[8] = s2;
+    }
+}
+
+template <bool FP, int R, bool OW>
+__global__ __launch_bounds__(KT_BLOCK) void k_matrix_ktab(const PtabArgs a) {
+    constexpr int Q = R / 4;                       // dwords per record
+    constexpr int SH = R == 8 ? 3 : 2;
+    constexpr uint32_t TAB = 256u * R;             // bytes per position table
+    constexpr uint32_t PER = KT_E * TAB;           // bytes per source
+    constexpr int STAGE = kt_stage<R>();
+    extern __shared__ uint4 s_pt[];
+    const uint8_t *lds = (const uint8_t *)s_pt;
+    const uint32_t b = blockIdx.x;
+    const uint32_t rg = __builtin_amdgcn_readfirstlane(b % a.ngroups);
+    const uint32_t ct = __builtin_amdgcn_readfirstlane(b / a.ngroups);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint64_t tile = ((uint64_t)ct * (KT_BLOCK / 64) + wave) * 1024u;
+    // Waves past the columns keep running with empty resources: every wave
+    // takes part in the staging barriers.
+    const uint32_t nb = tile >= a.ncols ? 0u : (uint32_t)std::min<uint64_t>(1024u, a.ncols - tile);
+    const uint64_t base = a.col0 + tile;
+    const uint32_t lo = lane * 16u;
+    const uint32_t j0 = rg * R;
+    const uint32_t mrows = std::min<uint32_t>(R, a.m - j0);
+    const uint8_t *pt = a.ptab + rg * a.group_stride;
+
+    uint32_t acc[4][KT_E][Q];
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+#pragma unroll
+        for (int p = 0; p < KT_E; ++p)
+#pragma unroll
+            for (int q = 0; q < Q; ++q) acc[e][p][q] = 0;
+
+    auto load_src = [&](uint32_t (&x)[4], uint32_t t) {
+        const bool real = t < a.k;
+        bload16(x, buf_res(real ? (const uint8_t *)a.src[t] + base : nullptr, real ? nb : 0u), lo);
+    };
+    uint32_t xc[4], xn[4];
+    load_src(xc, 0);
+    for (uint32_t t0 = 0; t0 < a.k; t0 += STAGE) {
+        const uint32_t kn = std::min<uint32_t>(STAGE, a.k - t0);
+        {
+            const uint4 *g = (const uint4 *)(pt + (size_t)t0 * PER);
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < kn * (PER / 16u); i += KT_BLOCK) s_pt[i] = g[i];
+            __syncthreads();
+        }
+#pragma unroll
+        for (int tl = 0; tl < STAGE; ++tl) {
+            if ((uint32_t)tl < kn) {
+                load_src(xn, t0 + tl + 1);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    uint32_t x[KT_E];
+                    kt_expand<FP, SH>(xc[e], x);
+#pragma unroll
+                    for (int p = 0; p < KT_E; ++p) {
+                        const uint8_t *rec = lds + (uint32_t)tl * PER + (uint32_t)p * TAB + x[p];
+                        if constexpr (R == 8) {
+                            const uint2 v = *(const uint2 *)rec;
+                            acc[e][p][0] ^= v.x;
+                            acc[e][p][1] ^= v.y;
+                        } else {
+                            acc[e][p][0] ^= *(const uint32_t *)rec;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int d = 0; d < 4; ++d) xc[d] = xn[d];
+            }
+        }
+    }
+
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        // out[e][i]: element e's bytes for rows 4q..4q+3 after the transpose.
+        uint32_t out[4][4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const uint32_t *P[KT_E];
+#pragma unroll
+            for (int p = 0; p < KT_E; ++p) P[p] = &acc[e][p][q];
+            uint32_t y0 = xor3(*P[0], *P[1], *P[3]) ^ *P[4];
+            uint32_t y1 = xor3(*P[0], *P[2], *P[3]) ^ *P[5];
+            uint32_t y2 = xor3(*P[0], *P[1], *P[6]) ^ *P[7];
+            uint32_t y3 = xor3(*P[0], *P[2], *P[6]) ^ *P[8];
+            transpose4(y0, y1, y2, y3);
+            out[e][0] = y0;
+            out[e][1] = y1;
+            out[e][2] = y2;
+            out[e][3] = y3;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t j = 4 * q + i;
+            if (j >= mrows) continue;
+            uint32_t w[4] = {out[0][i], out[1][i], out[2][i], out[3][i]};
+            const BufRes r = buf_res(a.dst[j0 + j] + base, nb);
+            if constexpr (!OW) {
+                uint32_t d[4];
+                bload16(d, r, lo);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) w[e] ^= d[e];
+            }
+            bstore16(r, lo, w);
+        }
+    }
+}
+
 // The packed-table launches of one column segment (all pointers + col0
 // 16-byte aligned, ncols a multiple of 16): sources k0..k0+kn of the plan,
 // rows j0..j0+mn (j0 a multiple of 16), groups of R rows.
@@ -633,7 +811,9 @@ hipError_t launch_ptab_segment(const MatrixJob &job, size_t k0, size_t kn, size_
     PtabArgs a{};
     for (size_t t = 0; t < kn; ++t) a.src[t] = job.srcs[k0 + t];
     for (size_t j = 0; j < mn; ++j) a.dst[j] = job.row(j0 + j);
-    const uint64_t per_source = (uint64_t)job.nb * 256u * job.nb * R;   // nb tables of nb * R-byte records
+    // nb tables of nb * R-byte records (1- and 2-byte fields), or the 9
+    // position tables of R-byte records (4-byte fields).
+    const uint64_t per_source = job.nb == 4 ? (uint64_t)KT_E * 256u * R : (uint64_t)job.nb * 256u * job.nb * R;
     a.group_stride = (uint64_t)job.ptab_k * per_source;
     a.ptab = job.ptab + (j0 / R) * a.group_stride + k0 * per_source;
     a.col0 = col0;
@@ -641,19 +821,31 @@ hipError_t launch_ptab_segment(const MatrixJob &job, size_t k0, size_t kn, size_
     a.k = (uint32_t)kn;
     a.m = (uint32_t)mn;
     a.ngroups = (uint32_t)((mn + R - 1) / R);
-    a.ncoltiles = (uint32_t)((ncols + PTAB_BLOCK * 16 - 1) / (PTAB_BLOCK * 16));
+    const uint32_t block = job.nb == 4 ? KT_BLOCK : PTAB_BLOCK;
+    a.ncoltiles = (uint32_t)((ncols + block * 16 - 1) / (block * 16));
     const uint64_t blocks = (uint64_t)a.ncoltiles * a.ngroups;
     using Fn = void (*)(PtabArgs);
     Fn fn;
-    if (job.nb == 1)
+    size_t lds = PTAB_LDS;
+    if (job.nb == 1) {
         fn = R == 16  ? (ow ? (Fn)k_matrix_ptab<16, true> : (Fn)k_matrix_ptab<16, false>)
              : R == 8 ? (ow ? (Fn)k_matrix_ptab<8, true> : (Fn)k_matrix_ptab<8, false>)
                       : (ow ? (Fn)k_matrix_ptab<4, true> : (Fn)k_matrix_ptab<4, false>);
-    else
+    } else if (job.nb == 2) {
         fn = R == 8   ? (ow ? (Fn)k_matrix_ptab2<8, true> : (Fn)k_matrix_ptab2<8, false>)
              : R == 4 ? (ow ? (Fn)k_matrix_ptab2<4, true> : (Fn)k_matrix_ptab2<4, false>)
                       : (ow ? (Fn)k_matrix_ptab2<2, true> : (Fn)k_matrix_ptab2<2, false>);
-    hipLaunchKernelGGL(fn, dim3((uint32_t)blocks), dim3(PTAB_BLOCK), PTAB_LDS, job.stream, a);
+    } else {
+        const bool fp = job.fan_paar;
+        if (R == 8)
+            fn = fp ? (ow ? (Fn)k_matrix_ktab<true, 8, true> : (Fn)k_matrix_ktab<true, 8, false>)
+                    : (ow ? (Fn)k_matrix_ktab<false, 8, true> : (Fn)k_matrix_ktab<false, 8, false>);
+        else
+            fn = fp ? (ow ? (Fn)k_matrix_ktab<true, 4, true> : (Fn)k_matrix_ktab<true, 4, false>)
+                    : (ow ? (Fn)k_matrix_ktab<false, 4, true> : (Fn)k_matrix_ktab<false, 4, false>);
+        lds = ktab_lds(R);
+    }
+    hipLaunchKernelGGL(fn, dim3((uint32_t)blocks), dim3(block), lds, job.stream, a);
     return hipGetLastError();
 }
 
@@ -695,7 +887,7 @@ hipError_t launch_matrix(const MatrixJob &job) {
             };
             hipError_t e;
             if ((e = segments(0, head, false)) != hipSuccess) return e;
-            if (job.ptab && (nb == 1 || nb == 2)) {
+            if (job.ptab && (nb == 1 || nb == 2 || nb == 4)) {
                 for (uint64_t c = 0; c < body; c += MAX_SEGMENT)
                     if ((e = launch_ptab_segment(job, k0, kn, j0, mn, head + c,
                                                  std::min<uint64_t>(MAX_SEGMENT, body - c), ow)) != hipSuccess)

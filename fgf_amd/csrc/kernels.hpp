@@ -121,15 +121,21 @@ FGF_HD constexpr void coeff_table(int f, uint64_t c, uint32_t *out) {
 // Host-side table builder: writes the entry_dwords(B) dwords of coefficient c.
 void build_coeff_table(int field, uint64_t c, uint32_t *out);
 
-// Packed product tables of a (k, m) plan over B-byte elements (B = 1 or 2):
-// rows in groups of R, and for group g, source t and byte position i < B a
-// 256-entry table whose record x (B * R bytes) holds the B-byte products
-// c(t, g*R + j) * (x << 8i) for j < R (zero past row m). Record x of
-// (g, t, i) is at byte (((g * k + t) * B + i) * 256 + x) * B * R. `vals` are
-// the plan values (coefficient (t, j) at t*m + j); `out` holds
-// packed_table_bytes(k, m, R, B).
+// Packed product tables of a (k, m) plan, rows in groups of R (zero past
+// row m). `vals` are the plan values (coefficient (t, j) at t*m + j); `out`
+// holds packed_table_bytes(k, m, R, B).
+//   B = 1 or 2: for group g, source t and byte position i < B a 256-entry
+//     table whose record x (B * R bytes) holds the B-byte products
+//     c(t, g*R + j) * (x << 8i) for j < R; record x of (g, t, i) at byte
+//     (((g * k + t) * B + i) * 256 + x) * B * R.
+//   B = 4: for group g, source t and Karatsuba position p < 9 a 256-entry
+//     table whose record x (R bytes) holds the byte products kc[p] * x of
+//     the 9 Karatsuba constants kc of c(t, g*R + j); record x of (g, t, p)
+//     at byte (((g * k + t) * 9 + p) * 256 + x) * R.
 inline size_t packed_table_bytes(size_t k, size_t m, int R, int B = 1) {
-    return ((m + R - 1) / R) * k * (size_t)B * 256 * (size_t)B * R;
+    const size_t groups = (m + R - 1) / R;
+    if (B == 4) return groups * k * 9 * 256 * (size_t)R;
+    return groups * k * (size_t)B * 256 * (size_t)B * R;
 }
 void build_packed_tables(int field, const uint64_t *vals, size_t k, size_t m, int R, uint8_t *out);
 
