@@ -8,8 +8,7 @@ persists the winner per (GPU arch, field, shape bucket) in a JSON file
 entry is reused without timing.
 
 `tune_plan` picks a plan's per-call kernel the same way: the v_perm form at
-several rows per block, and for 1-, 2- and 4-byte fields the packed 256-entry
-product tables, timed on a dot_product_matrix_with of the plan's shape, cached per
+several rows per block, and the packed 256-entry product tables, timed on a dot_product_matrix_with of the plan's shape, cached per
 (GPU arch, field, k, m, row-length bucket).
 """
 from __future__ import annotations
@@ -136,16 +135,15 @@ MAX_ROWS = {1: 16, 2: 8, 4: 4, 8: 8}
 def plan_candidates(nb: int, m: int = 1 << 30) -> list[tuple[int, int]]:
     """(form, rows per group) choices for an m-row plan: the v_perm form at
     its default row group (first), at the widest and at half the default,
-    dropping choices that split the m rows the same way, and for 1-, 2- and
-    4-byte fields the packed product tables."""
+    dropping choices that split the m rows the same way, and the packed
+    product tables."""
     default = DEFAULT_ROWS[nb]
     out, seen = [(0, 0)], {min(default, m)}
     for r in (MAX_ROWS[nb], default // 2):
         if r >= 1 and min(r, m) not in seen:
             seen.add(min(r, m))
             out.append((0, r))
-    if nb <= 4:
-        out.append((1, 0))
+    out.append((1, 0))
     return out
 
 

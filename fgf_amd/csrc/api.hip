@@ -615,16 +615,13 @@ void fgf_plan_free(fgf_plan *p) {
 int fgf_plan_set_kernel(fgf_plan *p, int form, int rows_per_group) {
     if (!p) return fail("Plan::set_kernel: plan is null");
     if (form != 0 && form != 1) return fail("Plan::set_kernel: form %d is not 0 (v_perm) or 1 (packed)", form);
-    if (form == 1 && p->nb > 4)
-        return fail("Plan::set_kernel: packed product tables are built for 1-, 2- and 4-byte fields, not %s",
-                    field_name(p->field));
     if (rows_per_group < 0 || rows_per_group > max_rows_for(p->nb))
         return fail("Plan::set_kernel: %d rows per group outside 0..%d for %s", rows_per_group,
                     max_rows_for(p->nb), field_name(p->field));
     if (form == 1 && !p->d_ptab && p->rows && p->cols) {
         // Records of at most 16 bytes: 16 / 8 / 4 rows of bytes, 8 / 4 / 2
         // rows of 2-byte elements; 8 / 4 rows of Karatsuba byte products for
-        // 4-byte elements.
+        // 4- and 8-byte elements.
         const int R = p->nb == 1   ? (p->cols > 8 ? 16 : p->cols > 4 ? 8 : 4)
                       : p->nb == 2 ? (p->cols > 4 ? 8 : p->cols > 2 ? 4 : 2)
                                    : (p->cols > 4 ? 8 : 4);

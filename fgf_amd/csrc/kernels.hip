@@ -27,18 +27,22 @@ void build_coeff_table(int field, uint64_t c, uint32_t *out) { coeff_table(field
 
 void build_packed_tables(int field, const uint64_t *vals, size_t k, size_t m, int R, uint8_t *out) {
     const size_t groups = (m + R - 1) / R;
-    if (field_bytes(field) == 4) {
-        // Nine tables per (group, source): Karatsuba position p, record x =
-        // the R byte products kc(t, j)[p] * x.
+    if (field_bytes(field) >= 4) {
+        // 9 (4-byte) or 27 (8-byte) tables per (group, source): Karatsuba
+        // position p, record x = the R byte products kc(t, j)[p] * x.
         const int bf = byte_field(field);
+        const size_t E = field_bytes(field) == 4 ? 9 : 27;
         for (size_t g = 0; g < groups; ++g)
             for (size_t t = 0; t < k; ++t)
                 for (int j = 0; j < R; ++j) {
                     const size_t row = g * R + j;
                     uint8_t kc[27] = {};
-                    if (row < m) kara_constants_of<32>(is_fan_paar(field), vals[t * m + row], kc);
-                    for (int p = 0; p < 9; ++p) {
-                        uint8_t *tab = out + ((g * k + t) * 9 + p) * 256u * R;
+                    if (row < m) {
+                        if (E == 9) kara_constants_of<32>(is_fan_paar(field), vals[t * m + row], kc);
+                        else kara_constants_of<64>(is_fan_paar(field), vals[t * m + row], kc);
+                    }
+                    for (size_t p = 0; p < E; ++p) {
+                        uint8_t *tab = out + ((g * k + t) * E + p) * 256u * R;
                         for (unsigned x = 0; x < 256; ++x) tab[x * R + j] = byte_mul(bf, kc[p], (uint8_t)x);
                     }
                 }
@@ -801,6 +805,147 @@ __global__ __launch_bounds__(KT_BLOCK) void k_matrix_ktab(const PtabArgs a) {
     }
 }
 
+// The packed form for 8-byte elements (GF(2^64), Fan-Paar 64): 27 position
+// tables per (group of R = 8 or 4 rows, source), record x = the R byte
+// products kc(t, j)[p] * x, p in the branch order of kara_constants_of<64>:
+// three GF(2^32) branches (towers: a, b, a + b with a, b the element's low
+// and high dwords; Fan-Paar: a + b, a, b) of 9 positions each. Accumulating
+// 27 positions per element would take 27 * R / 4 VGPRs per element, so each
+// source's lookups are combined per branch at once: a branch's 9 records
+// give its 4 GF(2^32) product bytes for every row (the 4-byte combine of
+// k_matrix_ktab), XORed into acc[e][branch][byte]. After the last source
+//   bytes 0..3 = S ^ O1, bytes 4..7 = S ^ O2
+// per row, and two 4 x 4 byte transposes per row quartet move them into
+// rows. A lane's 16 column bytes are 2 elements.
+constexpr int KT8_E = 27;
+// The LDS stage: 108 KiB, two sources of 8-row records or four of 4-row.
+constexpr size_t KTAB8_LDS = 108 * 1024;
+
+template <int R>
+constexpr int kt8_stage() { return (int)(KTAB8_LDS / ((size_t)KT8_E * 256 * R)); }
+
+template <bool FP, int R, bool OW>
+__global__ __launch_bounds__(KT_BLOCK) void k_matrix_ktab8(const PtabArgs a) {
+    constexpr int Q = R / 4;                       // dwords per record
+    constexpr int SH = R == 8 ? 3 : 2;
+    constexpr uint32_t TAB = 256u * R;             // bytes per position table
+    constexpr uint32_t PER = KT8_E * TAB;          // bytes per source
+    constexpr int STAGE = kt8_stage<R>();
+    extern __shared__ uint4 s_pt[];
+    const uint8_t *lds = (const uint8_t *)s_pt;
+    const uint32_t b = blockIdx.x;
+    const uint32_t rg = __builtin_amdgcn_readfirstlane(b % a.ngroups);
+    const uint32_t ct = __builtin_amdgcn_readfirstlane(b / a.ngroups);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint64_t tile = ((uint64_t)ct * (KT_BLOCK / 64) + wave) * 1024u;
+    // Waves past the columns keep running with empty resources: every wave
+    // takes part in the staging barriers.
+    const uint32_t nb = tile >= a.ncols ? 0u : (uint32_t)std::min<uint64_t>(1024u, a.ncols - tile);
+    const uint64_t base = a.col0 + tile;
+    const uint32_t lo = lane * 16u;
+    const uint32_t j0 = rg * R;
+    const uint32_t mrows = std::min<uint32_t>(R, a.m - j0);
+    const uint8_t *pt = a.ptab + rg * a.group_stride;
+
+    // acc[e][br][i][q]: element e, branch br, GF(2^32) byte i, rows 4q..4q+3.
+    uint32_t acc[2][3][4][Q];
+#pragma unroll
+    for (int e = 0; e < 2; ++e)
+#pragma unroll
+        for (int br = 0; br < 3; ++br)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int q = 0; q < Q; ++q) acc[e][br][i][q] = 0;
+
+    auto load_src = [&](uint32_t (&x)[4], uint32_t t) {
+        const bool real = t < a.k;
+        bload16(x, buf_res(real ? (const uint8_t *)a.src[t] + base : nullptr, real ? nb : 0u), lo);
+    };
+    uint32_t xc[4], xn[4];
+    load_src(xc, 0);
+    for (uint32_t t0 = 0; t0 < a.k; t0 += STAGE) {
+        const uint32_t kn = std::min<uint32_t>(STAGE, a.k - t0);
+        {
+            const uint4 *g = (const uint4 *)(pt + (size_t)t0 * PER);
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < kn * (PER / 16u); i += KT_BLOCK) s_pt[i] = g[i];
+            __syncthreads();
+        }
+#pragma unroll
+        for (int tl = 0; tl < STAGE; ++tl) {
+            if ((uint32_t)tl < kn) {
+                load_src(xn, t0 + tl + 1);
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    const uint32_t wa = xc[2 * e], wb = xc[2 * e + 1], ws = wa ^ wb;
+                    // Branch sources in the order of kara_constants_of<64>.
+                    const uint32_t src3[3] = {FP ? ws : wa, FP ? wa : wb, FP ? wb : ws};
+#pragma unroll
+                    for (int br = 0; br < 3; ++br) {
+                        uint32_t x[KT_E];
+                        kt_expand<FP, SH>(src3[br], x);
+                        uint32_t P[KT_E][Q];
+#pragma unroll
+                        for (int p = 0; p < KT_E; ++p) {
+                            const uint8_t *rec = lds + (uint32_t)tl * PER + (uint32_t)(KT_E * br + p) * TAB + x[p];
+                            if constexpr (R == 8) {
+                                const uint2 v = *(const uint2 *)rec;
+                                P[p][0] = v.x;
+                                P[p][1] = v.y;
+                            } else {
+                                P[p][0] = *(const uint32_t *)rec;
+                            }
+                        }
+#pragma unroll
+                        for (int q = 0; q < Q; ++q) {
+                            uint32_t *y = &acc[e][br][0][0];
+                            y[0 * Q + q] = xor3(xor3(y[0 * Q + q], P[0][q], P[1][q]), P[3][q], P[4][q]);
+                            y[1 * Q + q] = xor3(xor3(y[1 * Q + q], P[0][q], P[2][q]), P[3][q], P[5][q]);
+                            y[2 * Q + q] = xor3(xor3(y[2 * Q + q], P[0][q], P[1][q]), P[6][q], P[7][q]);
+                            y[3 * Q + q] = xor3(xor3(y[3 * Q + q], P[0][q], P[2][q]), P[6][q], P[8][q]);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int d = 0; d < 4; ++d) xc[d] = xn[d];
+            }
+        }
+    }
+
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        // w[i][d]: row 4q + i, dword d of the lane's 16 bytes (element d / 2,
+        // low or high half).
+        uint32_t w[4][4];
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            uint32_t l0 = acc[e][0][0][q] ^ acc[e][1][0][q], l1 = acc[e][0][1][q] ^ acc[e][1][1][q];
+            uint32_t l2 = acc[e][0][2][q] ^ acc[e][1][2][q], l3 = acc[e][0][3][q] ^ acc[e][1][3][q];
+            uint32_t h0 = acc[e][0][0][q] ^ acc[e][2][0][q], h1 = acc[e][0][1][q] ^ acc[e][2][1][q];
+            uint32_t h2 = acc[e][0][2][q] ^ acc[e][2][2][q], h3 = acc[e][0][3][q] ^ acc[e][2][3][q];
+            transpose4(l0, l1, l2, l3);
+            transpose4(h0, h1, h2, h3);
+            w[0][2 * e] = l0; w[1][2 * e] = l1; w[2][2 * e] = l2; w[3][2 * e] = l3;
+            w[0][2 * e + 1] = h0; w[1][2 * e + 1] = h1; w[2][2 * e + 1] = h2; w[3][2 * e + 1] = h3;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t j = 4 * q + i;
+            if (j >= mrows) continue;
+            const BufRes r = buf_res(a.dst[j0 + j] + base, nb);
+            if constexpr (!OW) {
+                uint32_t d[4];
+                bload16(d, r, lo);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) w[i][c] ^= d[c];
+            }
+            bstore16(r, lo, w[i]);
+        }
+    }
+}
+
 // The packed-table launches of one column segment (all pointers + col0
 // 16-byte aligned, ncols a multiple of 16): sources k0..k0+kn of the plan,
 // rows j0..j0+mn (j0 a multiple of 16), groups of R rows.
@@ -811,9 +956,11 @@ hipError_t launch_ptab_segment(const MatrixJob &job, size_t k0, size_t kn, size_
     PtabArgs a{};
     for (size_t t = 0; t < kn; ++t) a.src[t] = job.srcs[k0 + t];
     for (size_t j = 0; j < mn; ++j) a.dst[j] = job.row(j0 + j);
-    // nb tables of nb * R-byte records (1- and 2-byte fields), or the 9
-    // position tables of R-byte records (4-byte fields).
-    const uint64_t per_source = job.nb == 4 ? (uint64_t)KT_E * 256u * R : (uint64_t)job.nb * 256u * job.nb * R;
+    // nb tables of nb * R-byte records (1- and 2-byte fields), or the 9 / 27
+    // position tables of R-byte records (4- / 8-byte fields).
+    const uint64_t per_source = job.nb == 8   ? (uint64_t)KT8_E * 256u * R
+                                : job.nb == 4 ? (uint64_t)KT_E * 256u * R
+                                              : (uint64_t)job.nb * 256u * job.nb * R;
     a.group_stride = (uint64_t)job.ptab_k * per_source;
     a.ptab = job.ptab + (j0 / R) * a.group_stride + k0 * per_source;
     a.col0 = col0;
@@ -821,7 +968,7 @@ hipError_t launch_ptab_segment(const MatrixJob &job, size_t k0, size_t kn, size_
     a.k = (uint32_t)kn;
     a.m = (uint32_t)mn;
     a.ngroups = (uint32_t)((mn + R - 1) / R);
-    const uint32_t block = job.nb == 4 ? KT_BLOCK : PTAB_BLOCK;
+    const uint32_t block = job.nb >= 4 ? KT_BLOCK : PTAB_BLOCK;
     a.ncoltiles = (uint32_t)((ncols + block * 16 - 1) / (block * 16));
     const uint64_t blocks = (uint64_t)a.ncoltiles * a.ngroups;
     using Fn = void (*)(PtabArgs);
@@ -835,7 +982,7 @@ hipError_t launch_ptab_segment(const MatrixJob &job, size_t k0, size_t kn, size_
         fn = R == 8   ? (ow ? (Fn)k_matrix_ptab2<8, true> : (Fn)k_matrix_ptab2<8, false>)
              : R == 4 ? (ow ? (Fn)k_matrix_ptab2<4, true> : (Fn)k_matrix_ptab2<4, false>)
                       : (ow ? (Fn)k_matrix_ptab2<2, true> : (Fn)k_matrix_ptab2<2, false>);
-    } else {
+    } else if (job.nb == 4) {
         const bool fp = job.fan_paar;
         if (R == 8)
             fn = fp ? (ow ? (Fn)k_matrix_ktab<true, 8, true> : (Fn)k_matrix_ktab<true, 8, false>)
@@ -844,6 +991,15 @@ hipError_t launch_ptab_segment(const MatrixJob &job, size_t k0, size_t kn, size_
             fn = fp ? (ow ? (Fn)k_matrix_ktab<true, 4, true> : (Fn)k_matrix_ktab<true, 4, false>)
                     : (ow ? (Fn)k_matrix_ktab<false, 4, true> : (Fn)k_matrix_ktab<false, 4, false>);
         lds = ktab_lds(R);
+    } else {
+        const bool fp = job.fan_paar;
+        if (R == 8)
+            fn = fp ? (ow ? (Fn)k_matrix_ktab8<true, 8, true> : (Fn)k_matrix_ktab8<true, 8, false>)
+                    : (ow ? (Fn)k_matrix_ktab8<false, 8, true> : (Fn)k_matrix_ktab8<false, 8, false>);
+        else
+            fn = fp ? (ow ? (Fn)k_matrix_ktab8<true, 4, true> : (Fn)k_matrix_ktab8<true, 4, false>)
+                    : (ow ? (Fn)k_matrix_ktab8<false, 4, true> : (Fn)k_matrix_ktab8<false, 4, false>);
+        lds = KTAB8_LDS;
     }
     hipLaunchKernelGGL(fn, dim3((uint32_t)blocks), dim3(block), lds, job.stream, a);
     return hipGetLastError();
@@ -887,7 +1043,7 @@ hipError_t launch_matrix(const MatrixJob &job) {
             };
             hipError_t e;
             if ((e = segments(0, head, false)) != hipSuccess) return e;
-            if (job.ptab && (nb == 1 || nb == 2 || nb == 4)) {
+            if (job.ptab) {
                 for (uint64_t c = 0; c < body; c += MAX_SEGMENT)
                     if ((e = launch_ptab_segment(job, k0, kn, j0, mn, head + c,
                                                  std::min<uint64_t>(MAX_SEGMENT, body - c), ow)) != hipSuccess)

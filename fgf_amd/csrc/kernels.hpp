@@ -128,13 +128,13 @@ void build_coeff_table(int field, uint64_t c, uint32_t *out);
 //     table whose record x (B * R bytes) holds the B-byte products
 //     c(t, g*R + j) * (x << 8i) for j < R; record x of (g, t, i) at byte
 //     (((g * k + t) * B + i) * 256 + x) * B * R.
-//   B = 4: for group g, source t and Karatsuba position p < 9 a 256-entry
-//     table whose record x (R bytes) holds the byte products kc[p] * x of
-//     the 9 Karatsuba constants kc of c(t, g*R + j); record x of (g, t, p)
-//     at byte (((g * k + t) * 9 + p) * 256 + x) * R.
+//   B = 4 or 8: for group g, source t and Karatsuba position p < E (9 or
+//     27) a 256-entry table whose record x (R bytes) holds the byte products
+//     kc[p] * x of the E Karatsuba constants kc of c(t, g*R + j); record x
+//     of (g, t, p) at byte (((g * k + t) * E + p) * 256 + x) * R.
 inline size_t packed_table_bytes(size_t k, size_t m, int R, int B = 1) {
     const size_t groups = (m + R - 1) / R;
-    if (B == 4) return groups * k * 9 * 256 * (size_t)R;
+    if (B >= 4) return groups * k * (B == 4 ? 9 : 27) * 256 * (size_t)R;
     return groups * k * (size_t)B * 256 * (size_t)B * R;
 }
 void build_packed_tables(int field, const uint64_t *vals, size_t k, size_t m, int R, uint8_t *out);

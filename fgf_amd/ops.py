@@ -142,7 +142,7 @@ class Plan:
     def set_kernel(self, form: int = 0, rows_per_group: int = 0) -> None:
         """Kernel choice for this plan's per-call matrix launches (no
         counterpart in the reference): form 0 = v_perm byte-constant tables,
-        1 = packed 256-entry product tables (1-, 2- and 4-byte fields); rows_per_group 0 =
+        1 = packed 256-entry product tables; rows_per_group 0 =
         the field's maximum. See fgf_plan_set_kernel in include/fgf_amd.h."""
         _check(_lib.lib().fgf_plan_set_kernel(self._h, int(form), int(rows_per_group)))
 

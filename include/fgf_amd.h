@@ -123,14 +123,14 @@ int fgf_plan_get_at(const fgf_plan *plan, size_t row, size_t col, uint64_t *valu
  * `_with` matrix, scatter and scattered forms, device and host buffers); no
  * counterpart in the reference, whose backend picks one form per CPU. The
  * autotuner (fgf_amd/autotune.py tune_plan) times the choices and sets the
- * fastest. form 0: v_perm byte-constant tables (the default). form 1 (1-,
- * 2- and 4-byte fields): packed 256-entry product tables, one random-index
- * LDS read per source byte carrying the products of a group of rows (16, 8
- * or 4 rows of bytes; 8, 4 or 2 rows of 2-byte elements), or for 4-byte
- * elements one read per Karatsuba byte of the element (9 per element, 8 or 4
+ * fastest. form 0: v_perm byte-constant tables (the default). form 1:
+ * packed 256-entry product tables, one random-index LDS read per source
+ * byte carrying the products of a group of rows (16, 8 or 4 rows of bytes;
+ * 8, 4 or 2 rows of 2-byte elements), or for 4- and 8-byte elements one
+ * read per Karatsuba byte of the element (E = 9 or 27 per element, 8 or 4
  * rows); selecting it builds and uploads them (k * ceil(m / R) * B * 256 *
- * B * R bytes, k * ceil(m / R) * 9 * 256 * R for 4-byte fields), kept with
- * the plan until it is freed.
+ * B * R bytes, k * ceil(m / R) * E * 256 * R for 4- and 8-byte fields),
+ * kept with the plan until it is freed.
  * rows_per_group: destination rows per block of the v_perm form, 1 up to
  * 16 / 8 / 4 / 8 for 1- / 2- / 4- / 8-byte fields, 0 for the default (16 /
  * 8 / 4 / 4). Set before the plan is shared: not concurrently with calls

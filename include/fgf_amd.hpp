@@ -144,7 +144,7 @@ class Plan {
 
     /* Kernel choice of the per-call matrix launches (no counterpart in the
      * reference; fgf_plan_set_kernel): form 0 v_perm tables, 1 packed
-     * product tables (1-, 2- and 4-byte fields); rows_per_group 0 = default. */
+     * product tables; rows_per_group 0 = default. */
     void set_kernel(int form, int rows_per_group = 0) {
         detail::check(fgf_plan_set_kernel(h_, form, rows_per_group));
     }

@@ -1,5 +1,5 @@
 """Kernel choices of a plan (fgf_plan_set_kernel): the packed 256-entry
-product-table form of 1-, 2- and 4-byte fields and narrower row groups of the v_perm form
+product-table form of every field and narrower row groups of the v_perm form
 give the oracle's bytes on every matrix shape that takes a plan, and the
 per-call autotuner picks among them with identical outputs.
 """
@@ -8,7 +8,8 @@ import pytest
 import torch
 
 import oracle_lib as ol
-from fgf_amd import BY_NAME, FanPaar8, FanPaar16, FanPaar32, Gf8B, Gf8D, Gf16, Gf32, Gf64, Panic, autotune, ops
+from fgf_amd import (BY_NAME, FanPaar8, FanPaar16, FanPaar32, FanPaar64, Gf8B, Gf8D, Gf16, Gf32, Gf64, Panic,
+                     autotune, ops)
 
 pytestmark = pytest.mark.gpu
 
@@ -38,7 +39,8 @@ SHAPES = [(1, 1), (3, 5), (10, 4), (17, 9), (5, 8), (5, 16), (64, 16), (70, 20),
 LENGTHS = [16, 1040, 3 * 4096 + 48, 1000, 5 * 8192 + 1040]
 
 
-@pytest.mark.parametrize("f", [Gf8B, Gf8D, FanPaar8, Gf16, FanPaar16, Gf32, FanPaar32], ids=lambda f: NAMES[f.id])
+@pytest.mark.parametrize("f", [Gf8B, Gf8D, FanPaar8, Gf16, FanPaar16, Gf32, FanPaar32, Gf64, FanPaar64],
+                         ids=lambda f: NAMES[f.id])
 @pytest.mark.parametrize("k,m", SHAPES)
 def test_packed_tables_match_oracle(f, k, m):
     vals = coeffs(f, k * m, 0x100 + k * m)
@@ -60,7 +62,7 @@ def test_packed_tables_match_oracle(f, k, m):
         assert np.array_equal(acc.cpu().numpy(), oracle(f, L, m, vals, srcs, init)), (k, m, L)
 
 
-@pytest.mark.parametrize("f", [Gf8D, Gf8B, Gf16, Gf32, FanPaar32], ids=lambda f: NAMES[f.id])
+@pytest.mark.parametrize("f", [Gf8D, Gf8B, Gf16, Gf32, FanPaar32, Gf64, FanPaar64], ids=lambda f: NAMES[f.id])
 def test_packed_tables_scattered_and_misaligned(f):
     k, m, L, gap = 20, 11, 4096 + 32, 48
     vals = coeffs(f, k * m, 0x400)
@@ -82,8 +84,8 @@ def test_packed_tables_scattered_and_misaligned(f):
         ol.mul_add_matrix_scattered(NAMES[f.id], want, L, starts, vals, srcs)
         assert np.array_equal(d.cpu().numpy(), want), overwrite
     # Misaligned sub-slices (a common offset of 3 bytes, 2 for 2-byte
-    # elements, 4 for 4-byte elements: head, vector body, tail).
-    o = {1: 3, 2: 2, 4: 4}[f.BYTES]
+    # elements, 4 and 8 for 4- and 8-byte elements: head, vector body, tail).
+    o = {1: 3, 2: 2, 4: 4, 8: 8}[f.BYTES]
     buf = [torch.from_numpy(np.concatenate([np.zeros(o, np.uint8), s])).cuda() for s in srcs]
     rows = torch.zeros(m * L + o, dtype=torch.uint8, device="cuda")
     ops.dot_product_matrix_with(f, rows[o:], L, m, plan, [b[o:] for b in buf])
@@ -106,9 +108,6 @@ def test_row_groups_match_default(f):
 
 
 def test_set_kernel_refuses_bad_choices():
-    plan64 = ops.Plan.matrix(Gf64, 2, 2, [1, 2, 3, 4])
-    with pytest.raises(Panic, match="packed product tables are built for 1-, 2- and 4-byte fields"):
-        plan64.set_kernel(1, 0)
     plan32 = ops.Plan.matrix(Gf32, 2, 2, [1, 2, 3, 4])
     with pytest.raises(Panic, match="rows per group outside 0..4"):
         plan32.set_kernel(0, 5)
