@@ -5,9 +5,11 @@ every output row:
 * RS(10+4) Gf8D encode over ~1 GiB of source (L = 107,374,144 B): encode,
   erase two data and two parity shards, rebuild them through the decode
   matrix, compare byte for byte;
-* k=64, m=16 rebuild with 4 MiB shards and a seeded erasure of 16 of 80;
-* every tower field at k=32, m=8 with 16 MiB sources: oracle slices and
-  linearity in the coefficients, rows(C) ^ rows(C') == rows(C ^ C');
+* k=64, m=16 rebuild with 4 MiB shards and a seeded erasure of 16 of 80,
+  through the v_perm and the packed product-table kernels;
+* every tower field at k=32, m=8 with 16 MiB sources, through both kernel
+  forms: oracle slices, linearity in the coefficients,
+  rows(C) ^ rows(C') == rows(C ^ C'), and the two forms byte for byte equal;
 * a square tower code at 16 MiB: decode(encode(x)) == x;
 * Gf64 mul_elementwise on 2^26 pairs: oracle slices, commutativity and
   distributivity over XOR.
@@ -49,12 +51,13 @@ def check_slices(field, vals, srcs, rows, L, m, width):
         assert np.array_equal(got, want), (field.NAME, off)
 
 
-def round_trip(field, k, m, L, erased, seed):
+def round_trip(field, k, m, L, erased, seed, form=0):
     code = torch.empty((k + m) * L, dtype=torch.uint8, device="cuda")
     code[:k * L] = rand(k * L, seed)
     shard = lambda i: code[i * L:(i + 1) * L]
     vals = codec.encode_plan_values(field, k, m)
     enc = ops.Plan.matrix(field, k, m, vals)
+    enc.set_kernel(form, 0)
     ops.dot_product_matrix_with(field, code[k * L:], L, m, enc, [shard(t) for t in range(k)])
     check_slices(field, vals, [shard(t) for t in range(k)], [shard(k + j) for j in range(m)],
                  L, m, 1 << 14)
@@ -63,6 +66,7 @@ def round_trip(field, k, m, L, erased, seed):
         shard(e).zero_()
     dmat, surv = codec.decode_matrix(field, k, m, erased)
     dplan = ops.Plan.matrix(field, k, len(erased), codec.plan_values(dmat))
+    dplan.set_kernel(form, 0)
     ops.mul_add_matrix_scattered_with(field, code, L, [e * L for e in erased], dplan,
                                       [shard(i) for i in surv])
     assert torch.equal(torch.cat([shard(e) for e in erased]), saved)
@@ -73,9 +77,10 @@ def test_rs10_4_headline_size_erase_and_rebuild():
     round_trip(Gf8D, 10, 4, L, [2, 7, 11, 13], seed=101)
 
 
-def test_k64_m16_4mib_seeded_erasure_round_trip():
+@pytest.mark.parametrize("form", [0, 1], ids=["v_perm", "packed"])
+def test_k64_m16_4mib_seeded_erasure_round_trip(form):
     erased = sorted(int(x) for x in np.random.default_rng(31).choice(80, 16, replace=False))
-    round_trip(Gf8D, 64, 16, 4 << 20, erased, seed=102)
+    round_trip(Gf8D, 64, 16, 4 << 20, erased, seed=102, form=form)
 
 
 @pytest.mark.parametrize("field", [Gf16, Gf32, Gf64, FanPaar16, FanPaar32, FanPaar64],
@@ -88,13 +93,20 @@ def test_tower_k32_m8_16mib_slices_and_linearity(field):
     top = 1 << min(field.BITS, 63)
     a = [int(v) for v in g.integers(1, top, k * m, dtype=np.uint64)]
     b = [int(v) for v in g.integers(1, top, k * m, dtype=np.uint64)]
-    out = []
-    for vals in (a, b, [x ^ y for x, y in zip(a, b)]):
-        rows = torch.empty(m * L, dtype=torch.uint8, device="cuda")
-        ops.dot_product_matrix_with(field, rows, L, m, ops.Plan.matrix(field, k, m, vals), srcs)
-        out.append(rows)
-    check_slices(field, a, srcs, [out[0][j * L:(j + 1) * L] for j in range(m)], L, m, 4096)
-    assert torch.equal(out[0] ^ out[1], out[2])
+    first = []
+    for form in (0, 1):
+        out = []
+        for vals in (a, b, [x ^ y for x, y in zip(a, b)]):
+            rows = torch.empty(m * L, dtype=torch.uint8, device="cuda")
+            plan = ops.Plan.matrix(field, k, m, vals)
+            plan.set_kernel(form, 0)
+            ops.dot_product_matrix_with(field, rows, L, m, plan, srcs)
+            out.append(rows)
+        check_slices(field, a, srcs, [out[0][j * L:(j + 1) * L] for j in range(m)], L, m, 4096)
+        assert torch.equal(out[0] ^ out[1], out[2]), form
+        first.append(out[0])
+        del out
+    assert torch.equal(first[0], first[1])
 
 
 @pytest.mark.parametrize("field", [Gf64, FanPaar32], ids=lambda f: NAMES[f])
