@@ -425,6 +425,24 @@ constexpr size_t PTAB_LDS = 32 * 1024;
 // Threads per block: the waves of a block share its staged tables.
 constexpr int PTAB_BLOCK = 512;
 
+// dst[0..MAXV) = the first n uint4 of src, zeros past them (n <= MAXV), by a
+// block of BS threads: every thread's loads (range-checked, so past n they
+// return zeros without a branch) are issued before its first LDS write, so a
+// stage costs one L2 round trip instead of one per iteration of a copy loop.
+template <int BS, uint32_t MAXV>
+__device__ __forceinline__ void stage_copy(uint4 *dst, const void *src, uint32_t n) {
+    constexpr int C = (int)((MAXV + BS - 1) / BS);
+    const BufRes r = buf_res(src, n * 16u);
+    uint32_t t[C][4];
+#pragma unroll
+    for (int c = 0; c < C; ++c) bload16<0>(t[c], r, (threadIdx.x + (uint32_t)c * BS) * 16u);
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const uint32_t i = threadIdx.x + (uint32_t)c * BS;
+        if ((c + 1) * BS <= (int)MAXV || i < MAXV) dst[i] = make_uint4(t[c][0], t[c][1], t[c][2], t[c][3]);
+    }
+}
+
 struct PtabArgs {
     const uint8_t *src[KMAX];
     uint8_t *dst[MMAX];
@@ -497,7 +515,7 @@ __global__ __launch_bounds__(PTAB_BLOCK) void k_matrix_ptab(const PtabArgs a) {
             const uint32_t kn = std::min<uint32_t>(STAGE, a.k - t);
             const uint4 *g = (const uint4 *)(pt + (size_t)t * 256u * R);
             __syncthreads();
-            for (uint32_t i = threadIdx.x; i < kn * 16u * R; i += PTAB_BLOCK) s_pt[i] = g[i];
+            stage_copy<PTAB_BLOCK, STAGE * 16u * R>(s_pt, g, kn * 16u * R);
             __syncthreads();
         }
         load_src(xn, t + 1);
@@ -612,7 +630,7 @@ __global__ __launch_bounds__(PTAB_BLOCK) void k_matrix_ptab2(const PtabArgs a) {
             const uint32_t kn = std::min<uint32_t>(STAGE, a.k - t);
             const uint4 *g = (const uint4 *)(pt + (size_t)t * 512u * RB);
             __syncthreads();
-            for (uint32_t i = threadIdx.x; i < kn * 32u * RB; i += PTAB_BLOCK) s_pt[i] = g[i];
+            stage_copy<PTAB_BLOCK, STAGE * 32u * RB>(s_pt, g, kn * 32u * RB);
             __syncthreads();
         }
         load_src(xn, t + 1);
@@ -740,7 +758,7 @@ __global__ __launch_bounds__(KT_BLOCK) void k_matrix_ktab(const PtabArgs a) {
         {
             const uint4 *g = (const uint4 *)(pt + (size_t)t0 * PER);
             __syncthreads();
-            for (uint32_t i = threadIdx.x; i < kn * (PER / 16u); i += KT_BLOCK) s_pt[i] = g[i];
+            stage_copy<KT_BLOCK, STAGE * (PER / 16u)>(s_pt, g, kn * (PER / 16u));
             __syncthreads();
         }
 #pragma unroll
@@ -870,7 +888,7 @@ __global__ __launch_bounds__(KT_BLOCK) void k_matrix_ktab8(const PtabArgs a) {
         {
             const uint4 *g = (const uint4 *)(pt + (size_t)t0 * PER);
             __syncthreads();
-            for (uint32_t i = threadIdx.x; i < kn * (PER / 16u); i += KT_BLOCK) s_pt[i] = g[i];
+            stage_copy<KT_BLOCK, STAGE * (PER / 16u)>(s_pt, g, kn * (PER / 16u));
             __syncthreads();
         }
 #pragma unroll
