@@ -4,15 +4,16 @@ One-shot calls whose tables fit in 1 KiB carry them in the launches' kernel
 arguments (no allocation); larger ones upload a stream-ordered temporary.
 Both sides of that boundary must give the oracle's bytes. Calls that need no
 per-call upload can be captured into a graph and replayed; each replay is
-checked against the oracle on fresh source bytes. A call that needs the
-upload is refused while its stream captures.
+checked against the oracle on fresh source bytes, for the v_perm and the
+packed product-table kernels and the elementwise products. A call that needs
+the upload is refused while its stream captures.
 """
 import numpy as np
 import pytest
 import torch
 
 import oracle_lib as ol
-from fgf_amd import BY_NAME, Gf8D, Gf16, Gf64, Panic, ops
+from fgf_amd import BY_NAME, Gf8D, Gf16, Gf32, Gf64, Panic, ops
 
 pytestmark = pytest.mark.gpu
 
@@ -126,3 +127,31 @@ def test_one_shot_upload_is_refused_while_capturing():
     g.replay()
     torch.cuda.synchronize()
     assert not rows.any()
+
+
+@pytest.mark.parametrize("f,k,m", [(Gf8D, 64, 16), (Gf16, 32, 8), (Gf32, 32, 8), (Gf64, 32, 8)],
+                         ids=["gf8d", "gf16", "gf32", "gf64"])
+def test_packed_table_plans_and_elementwise_capture_and_replay(f, k, m):
+    # The packed product-table kernels (LDS stages of the plan's tables) and
+    # the elementwise product of the same field, captured and replayed.
+    row_len = 48 << 10
+    flat = coeffs_for(f, k * m, 0xA00 + f.id)
+    plan = ops.Plan.matrix(f, k, m, flat)
+    plan.set_kernel(1, 0)
+    d_srcs = [torch.empty(row_len, dtype=torch.uint8, device="cuda") for _ in range(k)]
+    rows = torch.empty(m * row_len, dtype=torch.uint8, device="cuda")
+    prod = torch.empty(row_len, dtype=torch.uint8, device="cuda")
+    ops.dot_product_matrix_with(f, rows, row_len, m, plan, d_srcs)
+    ops.mul_elementwise(f, prod, d_srcs[0], d_srcs[1])
+    torch.cuda.synchronize()
+
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        ops.dot_product_matrix_with(f, rows, row_len, m, plan, d_srcs)
+        ops.mul_elementwise(f, prod, d_srcs[0], d_srcs[1])
+
+    def check(fresh, r):
+        assert np.array_equal(rows.cpu().numpy(), oracle_matrix(f, row_len, m, flat, fresh)), r
+        assert np.array_equal(prod.cpu().numpy(), ol.mul_elementwise(NAMES[f.id], fresh[0], fresh[1])), r
+
+    _replays_match(g, d_srcs, check, seed=0xB00 + f.id)
