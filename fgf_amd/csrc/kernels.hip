@@ -43,7 +43,12 @@ void build_packed_tables(int field, const uint64_t *vals, size_t k, size_t m, in
                     }
                     for (size_t p = 0; p < E; ++p) {
                         uint8_t *tab = out + ((g * k + t) * E + p) * 256u * R;
-                        for (unsigned x = 0; x < 256; ++x) tab[x * R + j] = byte_mul(bf, kc[p], (uint8_t)x);
+                        // Linear in x: built from the products with its 8 bits.
+                        uint8_t pw[8], prod[256];
+                        for (int b = 0; b < 8; ++b) pw[b] = byte_mul(bf, kc[p], (uint8_t)(1u << b));
+                        prod[0] = 0;
+                        for (unsigned x = 1; x < 256; ++x) prod[x] = prod[x & (x - 1)] ^ pw[__builtin_ctz(x)];
+                        for (unsigned x = 0; x < 256; ++x) tab[x * R + j] = prod[x];
                     }
                 }
         return;
@@ -59,10 +64,14 @@ void build_packed_tables(int field, const uint64_t *vals, size_t k, size_t m, in
                     const uint64_t c = row < m ? vals[t * m + row] : 0;
                     for (int i = 0; i < 2; ++i) {
                         uint8_t *tab = out + ((g * k + t) * 2 + i) * 256u * RB;
+                        // Linear in x: built from the products with its 8 bits.
+                        uint16_t pw[8], prod[256];
+                        for (int b = 0; b < 8; ++b) pw[b] = (uint16_t)HostField::mul(field, c, (uint64_t)1 << (8 * i + b));
+                        prod[0] = 0;
+                        for (unsigned x = 1; x < 256; ++x) prod[x] = prod[x & (x - 1)] ^ pw[__builtin_ctz(x)];
                         for (unsigned x = 0; x < 256; ++x) {
-                            const uint64_t p = HostField::mul(field, c, (uint64_t)x << (8 * i));
-                            tab[x * RB + 2 * j] = (uint8_t)p;
-                            tab[x * RB + 2 * j + 1] = (uint8_t)(p >> 8);
+                            tab[x * RB + 2 * j] = (uint8_t)prod[x];
+                            tab[x * RB + 2 * j + 1] = (uint8_t)(prod[x] >> 8);
                         }
                     }
                 }
