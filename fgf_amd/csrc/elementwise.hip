@@ -232,8 +232,9 @@ __device__ __forceinline__ void store_planes(const uint32_t *r, BufRes res, uint
 }
 
 // PF: the next step's operands are loaded while the current step is
-// multiplied (the 8-byte fields: one wave per SIMD, whose 512 registers hold
-// both steps, and no other wave on the SIMD to cover the load latency).
+// multiplied (an 8-byte field whose plain build fits one wave per SIMD: the
+// wave's 512 registers hold both steps, and no other wave on the SIMD covers
+// the load latency).
 template <int F, bool PF>
 __global__ __launch_bounds__(BS_BLOCK) void k_elementwise_bs(uint8_t *dst, const uint8_t *a, const uint8_t *b,
                                                               uint64_t nbytes) {
@@ -359,27 +360,38 @@ struct EwBody {
 };
 
 // Blocks of a bit-sliced kernel that one CU holds at once (its register
-// use decides), asked of the runtime once per field.
-unsigned bs_blocks_per_cu(EwFn fn, int field) {
-    static std::atomic<unsigned> cache[FIELD_COUNT];
-    unsigned n = cache[field].load(std::memory_order_relaxed);
+// use decides), asked of the runtime once per field and form.
+unsigned bs_blocks_per_cu(EwFn fn, int field, bool prefetch) {
+    static std::atomic<unsigned> cache[FIELD_COUNT][2];
+    std::atomic<unsigned> &slot = cache[field][prefetch ? 1 : 0];
+    unsigned n = slot.load(std::memory_order_relaxed);
     if (n == 0) {
         int blocks = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void *)fn, BS_BLOCK, 0) != hipSuccess ||
             blocks <= 0)
             blocks = 1;
         n = (unsigned)blocks;
-        cache[field].store(n, std::memory_order_relaxed);
+        slot.store(n, std::memory_order_relaxed);
     }
     return n;
 }
 
+// The 8-byte fields run without the register prefetch where that build
+// fits two waves per SIMD (two 256-thread blocks per CU): the second wave
+// then covers the loads and the first wave's issue gaps. Where it fits only
+// one, the prefetching build holds both steps in that wave's 512 registers.
 template <int F>
 EwBody ew_body() {
     if constexpr (field_bytes(F) >= 2) {
-        constexpr bool PF = field_bytes(F) == 8;
-        const EwFn fn = (EwFn)k_elementwise_bs<F, PF>;
-        return {fn, BS_BLOCK, 2048u * field_bytes(F), bs_blocks_per_cu(fn, F)};
+        const EwFn plain = (EwFn)k_elementwise_bs<F, false>;
+        const unsigned n = bs_blocks_per_cu(plain, F, false);
+        if constexpr (field_bytes(F) == 8) {
+            if (n < 2) {
+                const EwFn pf = (EwFn)k_elementwise_bs<F, true>;
+                return {pf, BS_BLOCK, 2048u * field_bytes(F), bs_blocks_per_cu(pf, F, true)};
+            }
+        }
+        return {plain, BS_BLOCK, 2048u * field_bytes(F), n};
     } else {
         return {(EwFn)k_elementwise<F>, EW_BLOCK, EW_TILE, EW_BLOCKS_PER_CU};
     }
