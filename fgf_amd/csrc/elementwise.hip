@@ -231,11 +231,10 @@ __device__ __forceinline__ void store_planes(const uint32_t *r, BufRes res, uint
     }
 }
 
-// PF: the next step's operands are loaded while the current step is
-// multiplied (an 8-byte field whose plain build fits one wave per SIMD: the
-// wave's 512 registers hold both steps, and no other wave on the SIMD covers
-// the load latency).
-template <int F, bool PF>
+// Each wave steps over the buffer by whole grids. The 8-byte fields fit
+// two waves per SIMD (at most 256 registers), so a second wave covers each
+// wave's load latency; the 2- and 4-byte fields hold 8 and 4.
+template <int F>
 __global__ __launch_bounds__(BS_BLOCK) void k_elementwise_bs(uint8_t *dst, const uint8_t *a, const uint8_t *b,
                                                               uint64_t nbytes) {
     constexpr int NB = field_bytes(F), N = 8 * NB;
@@ -243,39 +242,17 @@ __global__ __launch_bounds__(BS_BLOCK) void k_elementwise_bs(uint8_t *dst, const
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint64_t stride = (uint64_t)gridDim.x * (BS_BLOCK / 64) * TILE;
-    uint64_t tile = ((uint64_t)blockIdx.x * (BS_BLOCK / 64) + wave) * TILE;
-    if (tile >= nbytes) return;
-    auto bytes_at = [&](uint64_t t) -> uint32_t {
-        return t < nbytes ? (uint32_t)(nbytes - t < TILE ? nbytes - t : TILE) : 0u;
-    };
-    // A step past the end loads zeros from an empty resource.
-    auto load = [&](uint32_t *x, uint32_t *y, uint64_t t) {
-        const uint32_t n = bytes_at(t);
-        const uint64_t o = n ? t : 0u;
-        load_planes<NB, BUF_NT>(x, buf_res(a + o, n), lane);
-        load_planes<NB, BUF_NT>(y, buf_res(b + o, n), lane);
-    };
-    uint32_t nx[N], ny[N];
-    if constexpr (PF) load(nx, ny, tile);
-    do {
+    for (uint64_t tile = ((uint64_t)blockIdx.x * (BS_BLOCK / 64) + wave) * TILE; tile < nbytes; tile += stride) {
+        const uint32_t n = nbytes - tile < TILE ? (uint32_t)(nbytes - tile) : TILE;
         uint32_t x[N], y[N], o[N];
-        if constexpr (PF) {
-#pragma unroll
-            for (int i = 0; i < N; ++i) {
-                x[i] = nx[i];
-                y[i] = ny[i];
-            }
-            load(nx, ny, tile + stride);
-        } else {
-            load(x, y, tile);
-        }
+        load_planes<NB, BUF_NT>(x, buf_res(a + tile, n), lane);
+        load_planes<NB, BUF_NT>(y, buf_res(b + tile, n), lane);
         transpose_planes<NB>(x);
         transpose_planes<NB>(y);
         bs::field_mul<F>(x, y, o);
         transpose_planes<NB>(o);
-        store_planes<NB>(o, buf_res(dst + tile, bytes_at(tile)), lane);
-        tile += stride;
-    } while (tile < nbytes);
+        store_planes<NB>(o, buf_res(dst + tile, n), lane);
+    }
 }
 
 // One element per thread, byte loads, products through log/exp tables.
@@ -360,38 +337,26 @@ struct EwBody {
 };
 
 // Blocks of a bit-sliced kernel that one CU holds at once (its register
-// use decides), asked of the runtime once per field and form.
-unsigned bs_blocks_per_cu(EwFn fn, int field, bool prefetch) {
-    static std::atomic<unsigned> cache[FIELD_COUNT][2];
-    std::atomic<unsigned> &slot = cache[field][prefetch ? 1 : 0];
-    unsigned n = slot.load(std::memory_order_relaxed);
+// use decides), asked of the runtime once per field.
+unsigned bs_blocks_per_cu(EwFn fn, int field) {
+    static std::atomic<unsigned> cache[FIELD_COUNT];
+    unsigned n = cache[field].load(std::memory_order_relaxed);
     if (n == 0) {
         int blocks = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void *)fn, BS_BLOCK, 0) != hipSuccess ||
             blocks <= 0)
             blocks = 1;
         n = (unsigned)blocks;
-        slot.store(n, std::memory_order_relaxed);
+        cache[field].store(n, std::memory_order_relaxed);
     }
     return n;
 }
 
-// The 8-byte fields run without the register prefetch where that build
-// fits two waves per SIMD (two 256-thread blocks per CU): the second wave
-// then covers the loads and the first wave's issue gaps. Where it fits only
-// one, the prefetching build holds both steps in that wave's 512 registers.
 template <int F>
 EwBody ew_body() {
     if constexpr (field_bytes(F) >= 2) {
-        const EwFn plain = (EwFn)k_elementwise_bs<F, false>;
-        const unsigned n = bs_blocks_per_cu(plain, F, false);
-        if constexpr (field_bytes(F) == 8) {
-            if (n < 2) {
-                const EwFn pf = (EwFn)k_elementwise_bs<F, true>;
-                return {pf, BS_BLOCK, 2048u * field_bytes(F), bs_blocks_per_cu(pf, F, true)};
-            }
-        }
-        return {plain, BS_BLOCK, 2048u * field_bytes(F), n};
+        const EwFn fn = (EwFn)k_elementwise_bs<F>;
+        return {fn, BS_BLOCK, 2048u * field_bytes(F), bs_blocks_per_cu(fn, F)};
     } else {
         return {(EwFn)k_elementwise<F>, EW_BLOCK, EW_TILE, EW_BLOCKS_PER_CU};
     }
