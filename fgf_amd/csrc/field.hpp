@@ -126,8 +126,21 @@ FGF_HD constexpr uint64_t tower_delta_of(unsigned bits) {
     return bits == 16 ? 0x20u : bits == 32 ? 0x2000u : 0x20000000ull;
 }
 
+#if !defined(__HIP_DEVICE_COMPILE__)
+// An fp8 product on the host at run time, through log/exp tables computed at
+// compile time: the bit-level product recurses to single bits (~0.2 us each),
+// and an 8-byte constant's tables take 216 of them.
+inline uint8_t host_fp8_mul(uint8_t a, uint8_t b) {
+    static constexpr ByteLogTables t = make_log_tables(FP8);
+    return (a && b) ? t.exp[t.log[a] + t.log[b]] : 0;
+}
+#endif
+
 // Product in byte field bf from the bit-level definitions.
 FGF_HD constexpr uint8_t byte_mul(int bf, uint8_t a, uint8_t b) {
+#if !defined(__HIP_DEVICE_COMPILE__)
+    if (bf == FP8 && !__builtin_is_constant_evaluated()) return host_fp8_mul(a, b);
+#endif
     return bf == GF8D ? poly_mul8(a, b, 0x1D) : bf == GF8B ? poly_mul8(a, b, 0x1B) : (uint8_t)fp_mul_bits(a, b, 8);
 }
 
