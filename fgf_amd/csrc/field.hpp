@@ -70,12 +70,18 @@ FGF_HD constexpr uint8_t poly_mul8(uint8_t a, uint8_t b, uint8_t low) {
 
 FGF_HD constexpr uint64_t lowbits(unsigned bits) { return bits >= 64 ? ~0ull : ((1ull << bits) - 1); }
 
-// Multiplication by the level's tower generator, recursively (fan_paar.rs:68-77).
+// Multiplication by the level's tower generator (fan_paar.rs:68-77): with
+// v = lo + hi X at width 2h, alpha v = hi + (lo + alpha hi) X, where alpha hi
+// is the same map one level down on the high half. Taken bottom-up along
+// that chain of high halves (the top bit first), so device code needs no
+// call stack. v < 2^bits.
 FGF_HD constexpr uint64_t fp_alpha(uint64_t v, unsigned bits) {
-    if (bits == 1) return v & 1;
-    unsigned h = bits / 2;
-    uint64_t lo = v & lowbits(h), hi = v >> h;
-    return hi | ((lo ^ fp_alpha(hi, h)) << h);
+    uint64_t r = (v >> (bits - 1)) & 1;
+    for (unsigned n = 1; n < bits; n *= 2) {
+        const uint64_t vi = (v >> (bits - 2 * n)) & lowbits(2 * n);
+        r = (vi >> n) | (((vi & lowbits(n)) ^ r) << n);
+    }
+    return r;
 }
 
 // Canonical tower product down to single bits (fan_paar.rs:80-94).
@@ -136,12 +142,28 @@ inline uint8_t host_fp8_mul(uint8_t a, uint8_t b) {
 }
 #endif
 
-// Product in byte field bf from the bit-level definitions.
-FGF_HD constexpr uint8_t byte_mul(int bf, uint8_t a, uint8_t b) {
-#if !defined(__HIP_DEVICE_COMPILE__)
-    if (bf == FP8 && !__builtin_is_constant_evaluated()) return host_fp8_mul(a, b);
+#if defined(__HIPCC__)
+// The same tables in each translation unit's constant memory, for fp8
+// products in device code at run time (per-call coefficient tables built by
+// a kernel): the bit-level product recurses, which would give the kernel a
+// dynamic call stack.
+static __constant__ ByteLogTables c_fp8_logs = make_log_tables(FP8);
 #endif
-    return bf == GF8D ? poly_mul8(a, b, 0x1D) : bf == GF8B ? poly_mul8(a, b, 0x1B) : (uint8_t)fp_mul_bits(a, b, 8);
+
+// Product in byte field bf: the polynomial fields directly; fp8 from its
+// bit-level definition under constant evaluation and from the log/exp
+// tables at run time.
+FGF_HD constexpr uint8_t byte_mul(int bf, uint8_t a, uint8_t b) {
+    if (bf == GF8D) return poly_mul8(a, b, 0x1D);
+    if (bf == GF8B) return poly_mul8(a, b, 0x1B);
+    if (!__builtin_is_constant_evaluated()) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return (a && b) ? c_fp8_logs.exp[c_fp8_logs.log[a] + c_fp8_logs.log[b]] : 0;
+#else
+        return host_fp8_mul(a, b);
+#endif
+    }
+    return (uint8_t)fp_mul_bits(a, b, 8);
 }
 
 // GF(2^2^n) tower product over Gf8B, u^2 = u + DELTA (gf16.rs:143-153,

@@ -12,7 +12,7 @@ import pytest
 import torch
 
 import oracle_lib as ol
-from fgf_amd import BY_NAME, Gf8D, Gf16, Gf64, lib
+from fgf_amd import BY_NAME, FanPaar8, FanPaar64, Gf8D, Gf16, Gf64, lib
 from fgf_amd.field import Panic, _check
 
 NAMES = {f.id: n for n, f in BY_NAME.items()}
@@ -66,7 +66,10 @@ def run_case(field, k, m, row_len, gap, seed):
     (Gf8D, (64, 16, 64 << 10, 48), (10, 4, 4096 + 16, 16)),
     (Gf16, (32, 8, 16 << 10, 32), (3, 2, 1024 + 2, 6)),
     (Gf64, (24, 8, 8 << 10, 64), (2, 1, 520, 8)),
-], ids=["gf8d", "gf16", "gf64"])
+    # Fan-Paar tables are built on the device through the fp8 log/exp tables.
+    (FanPaar8, (32, 12, 16 << 10, 16), (5, 3, 2048 + 5, 7)),
+    (FanPaar64, (24, 8, 8 << 10, 64), (2, 1, 520, 8)),
+], ids=["gf8d", "gf16", "gf64", "fp8", "fp64"])
 def test_noalloc_matches_existing_entry_point_on_two_calls(field, first, second):
     if not torch.cuda.is_available():
         pytest.skip("no gfx950 device")
