@@ -13,6 +13,7 @@ the host.
 from __future__ import annotations
 
 import ctypes
+import threading
 from typing import Iterable, Sequence
 
 from . import _lib
@@ -83,6 +84,9 @@ def _coeff_bytes(field: Field, coeffs: Sequence[int]) -> ctypes.Array:
     return (ctypes.c_uint8 * max(len(raw), 1)).from_buffer_copy(raw or b"\0")
 
 
+_arrays = threading.local()
+
+
 def _ptr_array(bufs: Sequence) -> tuple[ctypes.Array, ctypes.Array]:
     # One pass, no intermediate tuples: this runs on every call of the
     # multi-source ops, where it is most of the Python mirror's own cost.
@@ -99,8 +103,17 @@ def _ptr_array(bufs: Sequence) -> tuple[ctypes.Array, ctypes.Array]:
             raise Panic("buffer must be a contiguous byte range")
         ptrs.append(b.data_ptr())
         lens.append(b.nbytes)
-    n = max(len(ptrs), 1)
-    return (ctypes.c_void_p * n)(*ptrs), (ctypes.c_size_t * n)(*lens)
+    # One pair of arrays per thread, refilled on every call (a third of the
+    # cost of building new ones): the library reads them only during the
+    # call, copying the pointers into the launches' arguments, and the
+    # count passed beside them bounds what it reads.
+    pair = getattr(_arrays, "pair", None)
+    if pair is None or len(pair[0]) < len(ptrs):
+        cap = max(len(ptrs), 64)
+        pair = _arrays.pair = ((ctypes.c_void_p * cap)(), (ctypes.c_size_t * cap)())
+    pair[0][:len(ptrs)] = ptrs
+    pair[1][:len(lens)] = lens
+    return pair
 
 
 # ---------------------------------------------------------------------------
